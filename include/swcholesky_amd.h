@@ -1,0 +1,133 @@
+/*
+ * swcholesky_amd: MI355X (gfx950) numeric phase of swCholesky's supernodal sparse
+ * Cholesky -- C ABI.
+ *
+ * Every entry point has C linkage and takes plain pointers and sizes.  Unless a
+ * function says otherwise, pointers are host pointers, arrays use the layouts of
+ * the reference's BCSC factor (software/swCholesky/common/def.h:123-210) and
+ * integer status codes are returned (SWC_OK == 0; see swc_last_error()).
+ */
+#ifndef SWCHOLESKY_AMD_H
+#define SWCHOLESKY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SWC_OK 0
+#define SWC_EINVAL -1   /* bad argument or inconsistent structure */
+#define SWC_ENOMEM -2   /* host or device allocation failed */
+#define SWC_EDEVICE -3  /* HIP runtime error */
+#define SWC_ENOTPOS -4  /* matrix not positive definite (see swc_factor's info) */
+#define SWC_ESTATE -5   /* call out of order (e.g. solve before factor) */
+
+/* Message of the calling thread's last failing call ("" if none). */
+const char* swc_last_error(void);
+
+/* ---- host symbolic analysis ---------------------------------------------
+ * Restates analyze_p2 (cholesky/sw_LSparsity.h:248-702) for a GIVEN ordering:
+ * elimination tree, column counts, colcount-weighted postorder, relaxed
+ * supernodes (nrelax/zrelax as sw_choleskyTest01.cpp:112-113 when NULL),
+ * supernodal row lists, BCSC pointers and pruned descendant sets
+ * (cholesky/Inspection_Prune.h:24-56).  A is the lower triangle in CSC with
+ * sorted rows; perm[new] = old.  The H-level partition (InspectionLevel_06.h)
+ * is not produced: the numeric phase schedules from the descendant sets. */
+typedef struct swc_symbolic swc_symbolic;
+
+int swc_analyze(int n, const int* Ap, const int* Ai, const int64_t* perm, const int* nrelax,
+                const double* zrelax, swc_symbolic** out);
+void swc_symbolic_free(swc_symbolic* sym);
+/* sizes[8] = {n, nsuper, ssize, xsize, maxSupWid, maxCol, nnz(tril A), |pruneSet|};
+ * *flops = sum_j ColCount[j]^2 (ColumnCount.h:489-499). */
+int swc_symbolic_sizes(const swc_symbolic* sym, int64_t* sizes, double* flops);
+
+enum swc_symbolic_array {
+  SWC_SYM_PERM = 0,      /* int32[n]    L->Perm                         */
+  SWC_SYM_COLCOUNT = 1,  /* int32[n]    L->ColCount                     */
+  SWC_SYM_PARENT = 2,    /* int32[n]    column elimination tree         */
+  SWC_SYM_SUPER = 3,     /* int32[nsuper+1]  L->super (blockSet)        */
+  SWC_SYM_SPARENT = 4,   /* int32[nsuper]    L->sParent                 */
+  SWC_SYM_COL2SUP = 5,   /* int32[n]    L->col2Sup                      */
+  SWC_SYM_PI = 6,        /* int64[nsuper+1]  L->pi                      */
+  SWC_SYM_S = 7,         /* int32[ssize]     L->s (Ls)                  */
+  SWC_SYM_I_PTR = 8,     /* int64[n+1]  L->i_ptr (Li_ptr)               */
+  SWC_SYM_P = 9,         /* int64[n+1]  L->p (lC)                       */
+  SWC_SYM_PRUNEPTR = 10, /* int32[nsuper+1]                             */
+  SWC_SYM_PRUNESET = 11, /* int32[|pruneSet|]                           */
+  SWC_SYM_A2P = 12,      /* int32[n+1]  tril(P A P^T) column pointers   */
+  SWC_SYM_A2I = 13,      /* int32[nnz]  tril(P A P^T) row indices       */
+  SWC_SYM_A2SRC = 14     /* int64[nnz]  entry of A each A2 entry comes from */
+};
+int swc_symbolic_copy(const swc_symbolic* sym, int which, void* dst);
+
+/* ---- reference numeric entry point ----------------------------------------
+ * Drop-in for sw_cholesky_left_par_05 (cholesky/sw_parallel_PB_Cholesky_05.h:52-66,
+ * PRUNE variant; called at examples/sw_choleskyTest01.cpp:223-232).  A2 = tril(P A
+ * P^T) in CSC (c, r, values); lValues (xsize doubles, zeroed by the caller) receives
+ * L in the BCSC layout.  Factors on the current HIP device.  The H-level schedule
+ * arguments are accepted and ignored.  Returns 1 on success and 0 on failure (non
+ * SPD or device error; the reference returns 1 even for non-SPD input,
+ * sw_parallel_PB_Cholesky_05.c:818).  timing[0] receives the device time (s). */
+int sw_cholesky_left_par_05(int n, int* c, int* r, double* values, size_t* lC, int* Ls,
+                            size_t* Li_ptr, double* lValues, int* blockSet, int supNo,
+                            double* timing, int* prunePtr, int* pruneSet, int nLevels,
+                            int* levelPtr, int* levelSet, int nPar, int* parPtr, int* partition,
+                            int chunk, int threads, int super_max, int col_max, double* nodCost);
+
+/* ---- handle API (structure uploaded once, refactor / solve many times) ------ */
+typedef struct swc_handle swc_handle;
+
+/* Binds to the current HIP device and creates the handle's stream. */
+int swc_create(swc_handle** h);
+int swc_destroy(swc_handle* h);
+
+/* Uploads the symbolic structure (arrays exactly as analyze_p2 returns them,
+ * sw_LSparsity.h:248-702) and the pattern of A2 = tril(P A P^T).  With a2src
+ * non-NULL, A2 entry e holds input value a2src[e] and swc_factor takes nvals
+ * values in that input order; otherwise it takes the A2 values in A2 order. */
+int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_t* lC,
+               const int* Ls, const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
+               const int* A2p, const int* A2i, const int64_t* a2src, int64_t nvals);
+/* Same, from the host analysis (values then in the analysed A's entry order, and
+ * the permutation set for solves in the original ordering). */
+int swc_upload_symbolic(swc_handle* h, const swc_symbolic* sym);
+/* perm[new] = old: solves take and return vectors in the original ordering.
+ * NULL clears it (solves then work in the factor's ordering). */
+int swc_set_perm(swc_handle* h, const int* perm);
+
+/* Numeric factorization.  values: host or device pointer (values_on_device).
+ * *info = 0, or k+1 for the first column k whose pivot is not positive
+ * (LAPACK convention); returns SWC_ENOTPOS in that case. */
+int swc_factor(swc_handle* h, const double* values, int values_on_device, int* info);
+/* Copies L (xsize doubles, BCSC layout) to host or device memory. */
+int swc_get_factor(swc_handle* h, double* lValues, int on_device);
+/* Solves A X = B in place for nrhs columns (leading dimension ldb >= n):
+ * permutation, forward L y = P b (blockedLsolve, triangularSolve/Triangular_BCSC.h:14-49),
+ * backward L^T z = y (no reference counterpart), inverse permutation. */
+int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device);
+/* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
+int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
+
+/* ---- planning introspection (host only, no reference counterpart) ----------
+ * Builds the panel/tile/task plan the GPU kernels run, without touching the
+ * device, for tools and tests.  sizes[6] = {panels, tiles, tasks, levels,
+ * solve tasks, xsize}; swc_plan_export copies array `which` (0 panels, 1 tiles,
+ * 2 tasks, 3 level->tile offsets, 4 diagonal tiles per level, 5 panels by level,
+ * 6 level->panel offsets, 7 solve tasks, 8 panel->solve-task offsets) as the
+ * packed structs of swcholesky_amd/csrc/plan.h. */
+typedef struct swc_plan swc_plan;
+int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
+                   const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
+                   swc_plan** out);
+void swc_plan_free(swc_plan* p);
+int swc_plan_sizes(const swc_plan* p, int64_t* sizes, double* flops);
+int swc_plan_export(const swc_plan* p, int which, void* dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWCHOLESKY_AMD_H */

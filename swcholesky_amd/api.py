@@ -1,0 +1,206 @@
+"""Host-side interface of the numeric phase, mirroring the reference's entry points.
+
+Reference interface this mirrors (all paths under software/swCholesky/):
+
+* ``analyze_p2`` (cholesky/sw_LSparsity.h:248-702) -> :func:`analyze_p2`, returning
+  a :class:`BCSC` with the reference's field names (super, s, i_ptr, p, Perm,
+  sParent, col2Sup, nsuper, ssize, xsize) plus prunePtr / pruneSet.
+* ``sw_cholesky_left_par_05`` (cholesky/sw_parallel_PB_Cholesky_05.h:52-66) ->
+  :func:`sw_cholesky_left_par_05`: same arguments, factors on the GPU, writes
+  lValues in place, returns 1 on success and 0 on failure.
+* ``blockedLsolve`` (triangularSolve/Triangular_BCSC.h:14-49) and the solve the
+  reference lacks -> :class:`Factor` (forward, backward, permutation, refinement).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+_SYM = {
+    "Perm": (0, "n", np.int32), "ColCount": (1, "n", np.int32), "parent": (2, "n", np.int32),
+    "super": (3, "ns1", np.int32), "sParent": (4, "ns", np.int32), "col2Sup": (5, "n", np.int32),
+    "pi": (6, "ns1", np.int64), "s": (7, "ssize", np.int32), "i_ptr": (8, "n1", np.int64),
+    "p": (9, "n1", np.int64), "prunePtr": (10, "ns1", np.int32),
+    "pruneSet": (11, "nprune", np.int32), "a2p": (12, "n1", np.int32),
+    "a2i": (13, "nnz", np.int32), "a2src": (14, "nnz", np.int64),
+}
+
+
+class BCSC:
+    """Symbolic supernodal factor as analyze_p2 returns it (def.h:123-210 names)."""
+
+    def __init__(self, handle, sizes, flops):
+        self._h = handle
+        self.n, self.nsuper, self.ssize, self.xsize, self.maxSupWid, self.maxCol, self.nnz, \
+            self.nprune = (int(v) for v in sizes)
+        self.flops = float(flops)  # sum_j ColCount[j]^2
+        dims = {"n": self.n, "n1": self.n + 1, "ns": self.nsuper, "ns1": self.nsuper + 1,
+                "ssize": self.ssize, "nprune": self.nprune, "nnz": self.nnz}
+        lib = _lib.lib()
+        for name, (which, dim, dt) in _SYM.items():
+            a = np.zeros(dims[dim], dtype=dt)
+            check(lib.swc_symbolic_copy(self._h, which, ptr(a)))
+            setattr(self, name, a)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().swc_symbolic_free(self._h)
+            self._h = None
+
+    def a2_values(self, values):
+        """Values of tril(P A P^T) in A2 order from values of A in input order."""
+        return np.ascontiguousarray(np.asarray(values, dtype=np.float64)[self.a2src])
+
+
+def analyze_p2(n, colptr, rowidx, perm, nrelax=(4, 16, 48), zrelax=(0.8, 0.1, 0.05)):
+    """Symbolic analysis for a GIVEN ordering (perm[new] = old)."""
+    colptr = np.ascontiguousarray(colptr, dtype=np.int32)
+    rowidx = np.ascontiguousarray(rowidx, dtype=np.int32)
+    perm = np.ascontiguousarray(perm, dtype=np.int64)
+    nr = np.asarray(nrelax, dtype=np.int32)
+    zr = np.asarray(zrelax, dtype=np.float64)
+    h = ctypes.c_void_p()
+    lib = _lib.lib()
+    check(lib.swc_analyze(int(n), ptr(colptr), ptr(rowidx), ptr(perm), ptr(nr), ptr(zr),
+                          ctypes.byref(h)))
+    sizes = np.zeros(8, dtype=np.int64)
+    fl = ctypes.c_double()
+    check(lib.swc_symbolic_sizes(h, ptr(sizes), ctypes.byref(fl)))
+    return BCSC(h, sizes, fl.value)
+
+
+def sw_cholesky_left_par_05(n, c, r, values, lC, Ls, Li_ptr, lValues, blockSet, supNo,
+                            timing, prunePtr, pruneSet, nLevels=0, levelPtr=None,
+                            levelSet=None, nPar=0, parPtr=None, partition=None, chunk=1,
+                            threads=1, super_max=0, col_max=0, nodCost=None):
+    """The reference numeric entry point on the GPU (host arrays, lValues in place)."""
+    arrs = [np.ascontiguousarray(a) for a in (c, r, values, lC, Ls, Li_ptr, blockSet,
+                                              prunePtr, pruneSet)]
+    c, r, values, lC, Ls, Li_ptr, blockSet, prunePtr, pruneSet = arrs
+    assert lValues.dtype == np.float64 and lValues.flags.c_contiguous
+    return _lib.lib().sw_cholesky_left_par_05(
+        int(n), ptr(c), ptr(r), ptr(values), ptr(lC), ptr(Ls), ptr(Li_ptr), ptr(lValues),
+        ptr(blockSet), int(supNo), ptr(timing), ptr(prunePtr), ptr(pruneSet), int(nLevels),
+        ptr(levelPtr), ptr(levelSet), int(nPar), ptr(parPtr), ptr(partition), int(chunk),
+        int(threads), int(super_max), int(col_max), ptr(nodCost))
+
+
+def _dptr(x):
+    """(pointer, on_device) of a numpy array or a torch tensor."""
+    if isinstance(x, np.ndarray):
+        return ptr(x), 0
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr()), int(x.is_cuda)
+    raise TypeError("expected a numpy array or a torch tensor")
+
+
+class Factor:
+    """GPU numeric factor of one symbolic structure (the handle API).
+
+    ``Factor(bcsc)`` uploads the structure once; :meth:`factor` takes the values of
+    A in the analysed input order (host numpy or device torch), :meth:`solve`
+    solves A x = b in the original ordering.
+    """
+
+    def __init__(self, bcsc: BCSC):
+        self.bcsc = bcsc
+        self.n = bcsc.n
+        lib = _lib.lib()
+        h = ctypes.c_void_p()
+        check(lib.swc_create(ctypes.byref(h)))
+        self._h = h
+        check(lib.swc_upload_symbolic(self._h, bcsc._h))
+        self.info = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().swc_destroy(self._h)
+            self._h = None
+
+    def factor(self, values):
+        """Numeric factorization; returns LAPACK-style info (0 = success)."""
+        if isinstance(values, np.ndarray):
+            values = np.ascontiguousarray(values, dtype=np.float64)
+        p, dev = _dptr(values)
+        info = ctypes.c_int(0)
+        rc = _lib.lib().swc_factor(self._h, p, dev, ctypes.byref(info))
+        self.info = info.value
+        if rc not in (0, -4):
+            check(rc)
+        return self.info
+
+    def L_values(self):
+        out = np.zeros(self.bcsc.xsize, dtype=np.float64)
+        check(_lib.lib().swc_get_factor(self._h, ptr(out), 0))
+        return out
+
+    def solve(self, b):
+        """x = A^{-1} b for a vector or an n x nrhs matrix (numpy, returns a copy)."""
+        b = np.asarray(b, dtype=np.float64)
+        vec = b.ndim == 1
+        X = np.asfortranarray(b.reshape(self.n, -1)).copy(order="F")
+        check(_lib.lib().swc_solve(self._h, ptr(X), X.shape[1], self.n, 0))
+        return X[:, 0].copy() if vec else X
+
+    def solve_device(self, B):
+        """In-place solve of a device tensor B (n x nrhs, column-major = B.T contiguous)."""
+        p, dev = _dptr(B)
+        nrhs = 1 if B.dim() == 1 else B.shape[0]
+        check(_lib.lib().swc_solve(self._h, p, nrhs, self.n, dev))
+        return B
+
+    def plan_stats(self):
+        out = np.zeros(5, dtype=np.int64)
+        fl = (ctypes.c_double * 2)()
+        check(_lib.lib().swc_plan_stats(self._h, ptr(out), fl))
+        return dict(panels=int(out[0]), tiles=int(out[1]), tasks=int(out[2]),
+                    levels=int(out[3]), xsize=int(out[4]), update_flops=fl[0],
+                    factor_flops=fl[1])
+
+
+PANEL_DT = np.dtype([("vbase", np.int64), ("ld", np.int32), ("ncol", np.int32),
+                     ("nrow", np.int32), ("c0", np.int32), ("rows_off", np.int32),
+                     ("sn", np.int32)])
+TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
+                    ("task_begin", np.int32), ("task_end", np.int32), ("kind", np.int32),
+                    ("pad0", np.int32), ("pad1", np.int32)])
+TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
+                    ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
+                    ("ca", np.int32), ("cb", np.int32), ("pad", np.int32)])
+STASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
+                     ("rows_off", np.int32), ("ca", np.int32), ("cb", np.int32),
+                     ("src_c0", np.int32)])
+
+
+def build_plan(bcsc: BCSC):
+    """The device work plan for a structure, as numpy record arrays (host only)."""
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    super_ = np.ascontiguousarray(bcsc.super)
+    p = np.ascontiguousarray(bcsc.p.astype(np.uint64))
+    iptr = np.ascontiguousarray(bcsc.i_ptr.astype(np.uint64))
+    check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
+                             ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
+    try:
+        sz = np.zeros(6, dtype=np.int64)
+        fl = (ctypes.c_double * 2)()
+        check(lib.swc_plan_sizes(h, ptr(sz), fl))
+        npanel, ntile, ntask, nlevel, nstask, xsize = (int(v) for v in sz)
+        spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
+                (3, np.int32, nlevel + 1), (4, np.int32, nlevel), (5, np.int32, npanel),
+                (6, np.int32, nlevel + 1), (7, STASK_DT, nstask), (8, np.int32, npanel + 1)]
+        names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_diag", "panel_order",
+                 "lvl_panels", "stasks", "stask_ptr"]
+        out = {}
+        for (which, dt, cnt), name in zip(spec, names):
+            a = np.zeros(cnt, dtype=dt)
+            check(lib.swc_plan_export(h, which, ptr(a)))
+            out[name] = a
+        out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1])
+        return out
+    finally:
+        lib.swc_plan_free(h)

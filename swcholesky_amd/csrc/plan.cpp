@@ -1,0 +1,258 @@
+// Builds the panel / tile / task plan (see plan.h).
+//
+// The per-descendant overlap scan of the reference (lb/ub in
+// sw_parallel_PB_Cholesky_05.c:247-264) becomes binary searches on the sorted
+// row lists, done once here instead of once per factorization.
+#include "plan.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace swc {
+
+namespace {
+
+int lower_idx(const int* a, int len, int v) { return int(std::lower_bound(a, a + len, v) - a); }
+int upper_idx(const int* a, int len, int v) { return int(std::upper_bound(a, a + len, v) - a); }
+
+}  // namespace
+
+void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
+                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P) {
+  P = Plan();
+  P.n = n;
+  P.nsuper = nsuper;
+  std::vector<int> first_panel(nsuper + 1, 0);
+  for (int s = 0; s < nsuper; ++s) {
+    const int W = super[s + 1] - super[s];
+    const int64_t R = pi[s + 1] - pi[s];
+    if (W <= 0 || R < W) throw std::invalid_argument("inconsistent supernode structure");
+    if (R > 0x7fffffff) throw std::invalid_argument("supernode too tall");
+    first_panel[s] = (int)P.panels.size();
+    for (int q = 0; q * PW < W; ++q) {
+      Panel p;
+      p.vbase = px[s] + (int64_t)q * PW * R + (int64_t)q * PW;
+      p.ld = (int)R;
+      p.ncol = std::min(PW, W - q * PW);
+      p.nrow = (int)R - q * PW;
+      p.c0 = super[s] + q * PW;
+      p.rows_off = (int)(pi[s] + q * PW);
+      p.sn = s;
+      P.panels.push_back(p);
+    }
+    P.xsize = std::max<int64_t>(P.xsize, px[s] + (int64_t)W * R);
+  }
+  first_panel[nsuper] = (int)P.panels.size();
+  P.npanel = (int)P.panels.size();
+
+  // ---- sources of each panel, and levels ----
+  struct Src { int d; int ca, cb; };  // d >= 0: supernode, d < 0: panel -(d+1) of same sn
+  std::vector<std::vector<Src>> srcs(P.npanel);
+  P.panel_level.assign(P.npanel, 0);
+  std::vector<int> sn_level(nsuper, -1);  // level of the last panel of each supernode
+  for (int s = 0; s < nsuper; ++s) {
+    const int fp = first_panel[s], lp = first_panel[s + 1];
+    for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) {
+      const int d = pruneSet[t];
+      if (d < 0 || d >= s) throw std::invalid_argument("pruneSet entry is not a descendant");
+      const int* rows = Ls + pi[d];
+      const int len = (int)(pi[d + 1] - pi[d]);
+      for (int q = fp; q < lp; ++q) {
+        const Panel& pn = P.panels[q];
+        const int ca = lower_idx(rows, len, pn.c0);
+        const int cb = lower_idx(rows, len, pn.c0 + pn.ncol);
+        if (ca < cb) {
+          srcs[q].push_back({d, ca, cb});
+          P.panel_level[q] = std::max(P.panel_level[q], sn_level[d] + 1);
+        }
+      }
+    }
+    for (int q = fp; q < lp; ++q) {
+      for (int q2 = fp; q2 < q; ++q2) {
+        const int off = (q - fp) * PW;
+        srcs[q].push_back({-(q2 + 1), off, off + P.panels[q].ncol});
+      }
+      if (q > fp) P.panel_level[q] = std::max(P.panel_level[q], P.panel_level[q - 1] + 1);
+    }
+    sn_level[s] = P.panel_level[lp - 1];
+  }
+  P.nlevel = 0;
+  for (int q = 0; q < P.npanel; ++q) P.nlevel = std::max(P.nlevel, P.panel_level[q] + 1);
+
+  // ---- panels grouped by level ----
+  P.lvl_panels.assign(P.nlevel + 1, 0);
+  for (int q = 0; q < P.npanel; ++q) P.lvl_panels[P.panel_level[q] + 1]++;
+  for (int l = 0; l < P.nlevel; ++l) P.lvl_panels[l + 1] += P.lvl_panels[l];
+  P.panel_order.resize(P.npanel);
+  {
+    std::vector<int> fill(P.lvl_panels.begin(), P.lvl_panels.end() - 1);
+    for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
+  }
+
+  // ---- source descriptors: task prototypes shared by the tiles of a panel ----
+  auto describe = [&](int q, const Src& sr, Task& t, SolveTask& st) {
+    const Panel& pn = P.panels[q];
+    if (sr.d >= 0) {
+      const int d = sr.d;
+      t.src_base = px[d];
+      t.ld = (int)(pi[d + 1] - pi[d]);
+      t.k = super[d + 1] - super[d];
+      t.rows_off = (int)pi[d];
+      st.src_c0 = super[d];
+    } else {
+      const Panel& sp = P.panels[-(sr.d + 1)];
+      const int s = pn.sn;
+      const int qoff = sp.c0 - super[s];
+      t.src_base = px[s] + (int64_t)qoff * pn.ld;
+      t.ld = pn.ld;
+      t.k = sp.ncol;
+      t.rows_off = (int)pi[s];
+      st.src_c0 = sp.c0;
+    }
+    t.ca = sr.ca;
+    t.cb = sr.cb;
+    t.pad = 0;
+    st.src_base = t.src_base;
+    st.ld = t.ld;
+    st.k = t.k;
+    st.rows_off = t.rows_off;
+    st.ca = sr.ca;
+    st.cb = sr.cb;
+  };
+
+  // forward-solve sources
+  P.stask_ptr.assign(P.npanel + 1, 0);
+  for (int q = 0; q < P.npanel; ++q) {
+    for (const Src& sr : srcs[q]) {
+      Task t;
+      SolveTask st;
+      describe(q, sr, t, st);
+      P.stasks.push_back(st);
+    }
+    P.stask_ptr[q + 1] = (int)P.stasks.size();
+  }
+
+  // ---- tiles and tasks, level by level (diagonal tiles first in a level) ----
+  P.lvl_tiles.assign(P.nlevel + 1, 0);
+  P.lvl_diag.assign(P.nlevel, 0);
+  for (int l = 0; l < P.nlevel; ++l) {
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int qi = P.lvl_panels[l]; qi < P.lvl_panels[l + 1]; ++qi) {
+        const int q = P.panel_order[qi];
+        const Panel& pn = P.panels[q];
+        for (int r0 = pass == 0 ? 0 : TM; r0 < (pass == 0 ? 1 : pn.nrow); r0 += TM) {
+          Tile tl;
+          tl.panel = q;
+          tl.r0 = r0;
+          tl.nr = std::min(TM, pn.nrow - r0);
+          tl.kind = pass;
+          tl.pad0 = tl.pad1 = 0;
+          tl.task_begin = (int)P.tasks.size();
+          const int vfirst = Ls[pn.rows_off + r0];
+          const int vlast = Ls[pn.rows_off + r0 + tl.nr - 1];
+          for (const Src& sr : srcs[q]) {
+            Task t;
+            SolveTask st;
+            describe(q, sr, t, st);
+            const int* rows = Ls + t.rows_off;
+            int ra, rb;
+            if (sr.d >= 0) {
+              const int len = (int)(pi[sr.d + 1] - pi[sr.d]);
+              ra = std::max(lower_idx(rows, len, vfirst), sr.ca);
+              rb = upper_idx(rows, len, vlast);
+            } else {
+              ra = (pn.c0 - super[pn.sn]) + r0;
+              rb = ra + tl.nr;
+            }
+            if (ra >= rb) continue;
+            t.ra = ra;
+            t.rb = rb;
+            P.tasks.push_back(t);
+            P.update_flops += 2.0 * (rb - ra) * (double)(t.cb - t.ca) * t.k;
+          }
+          tl.task_end = (int)P.tasks.size();
+          P.tiles.push_back(tl);
+          if (pass == 0) P.lvl_diag[l]++;
+        }
+      }
+    }
+    P.lvl_tiles[l + 1] = (int)P.tiles.size();
+  }
+  for (const Panel& pn : P.panels) {
+    const double w = pn.ncol, m = pn.nrow - pn.ncol;
+    P.factor_flops += w * w * w / 3.0 + m * w * w;
+  }
+}
+
+}  // namespace swc
+
+// ---- host-only planning introspection (C ABI) ----
+#include "capi_common.h"
+
+struct swc_plan {
+  swc::Plan P;
+};
+
+extern "C" {
+
+int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
+                   const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
+                   swc_plan** out) {
+  if (!out || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr || !pruneSet || supNo <= 0)
+    return swc_set_error(SWC_EINVAL, "swc_plan_build: bad argument");
+  *out = nullptr;
+  try {
+    std::vector<int64_t> pi(supNo + 1), px(supNo);
+    for (int s = 0; s <= supNo; ++s) pi[s] = (int64_t)Li_ptr[blockSet[s]];
+    for (int s = 0; s < supNo; ++s) px[s] = (int64_t)lC[blockSet[s]];
+    auto* h = new swc_plan;
+    swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P);
+    *out = h;
+    return SWC_OK;
+  } catch (const std::exception& e) {
+    return swc_set_error(SWC_EINVAL, std::string("swc_plan_build: ") + e.what());
+  }
+}
+
+void swc_plan_free(swc_plan* p) { delete p; }
+
+int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
+  if (!p || !out) return swc_set_error(SWC_EINVAL, "swc_plan_sizes: null argument");
+  const swc::Plan& P = p->P;
+  out[0] = P.npanel;
+  out[1] = (int64_t)P.tiles.size();
+  out[2] = (int64_t)P.tasks.size();
+  out[3] = P.nlevel;
+  out[4] = (int64_t)P.stasks.size();
+  out[5] = P.xsize;
+  if (flops) {
+    flops[0] = P.update_flops;
+    flops[1] = P.factor_flops;
+  }
+  return SWC_OK;
+}
+
+int swc_plan_export(const swc_plan* p, int which, void* dst) {
+  if (!p || !dst) return swc_set_error(SWC_EINVAL, "swc_plan_export: null argument");
+  const swc::Plan& P = p->P;
+  auto cp = [&](const auto& v) {
+    std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    return SWC_OK;
+  };
+  switch (which) {
+    case 0: return cp(P.panels);
+    case 1: return cp(P.tiles);
+    case 2: return cp(P.tasks);
+    case 3: return cp(P.lvl_tiles);
+    case 4: return cp(P.lvl_diag);
+    case 5: return cp(P.panel_order);
+    case 6: return cp(P.lvl_panels);
+    case 7: return cp(P.stasks);
+    case 8: return cp(P.stask_ptr);
+    default: return swc_set_error(SWC_EINVAL, "swc_plan_export: unknown array id");
+  }
+}
+
+}  // extern "C"

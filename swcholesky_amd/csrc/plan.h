@@ -1,0 +1,83 @@
+// Device work plan of the numeric phase, built once per symbolic structure.
+//
+// Every supernode is cut into panels of at most PW consecutive columns; a panel
+// is a target of the left-looking factorization.  The sources of a panel are the
+// reference's pruned descendants of its supernode that have rows in the panel's
+// columns (in pruneSet order) followed by the earlier panels of the same
+// supernode.  Each panel's rows are cut into tiles of TM rows; a tile owns its
+// accumulator, so every entry of L is updated by exactly one workgroup, in a
+// fixed source order, without atomics.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace swc {
+
+constexpr int PW = 64;  // panel width (columns)
+constexpr int TM = 64;  // tile height (rows)
+
+struct Panel {
+  int64_t vbase;  // value offset of (panel row 0, panel column 0)
+  int ld;         // leading dimension (row count of the supernode)
+  int ncol;       // columns in the panel (<= PW)
+  int nrow;       // rows from the panel's first column down
+  int c0;         // first global column
+  int rows_off;   // offset into Ls of the panel's row 0
+  int sn;         // owning supernode
+};
+
+struct Tile {
+  int panel;
+  int r0;          // first panel row of the tile
+  int nr;          // rows in the tile (<= TM)
+  int task_begin;  // [task_begin, task_end) into the task array
+  int task_end;
+  int kind;        // 0: diagonal tile (POTRF + own TRSM), 1: off-diagonal
+  int pad0, pad1;
+};
+
+// One source contribution to one tile: rows [ra,rb) x columns [ca,cb) of the
+// source's row list, inner dimension k.
+struct Task {
+  int64_t src_base;  // value offset of (source row 0, source column 0)
+  int ld;            // source leading dimension
+  int k;             // source column count
+  int rows_off;      // offset into Ls of the source's row 0
+  int ra, rb;        // source rows landing in the tile's rows
+  int ca, cb;        // source rows landing in the panel's columns
+  int pad;
+};
+
+// Forward/backward solve contributions into one panel: source rows [ca,cb)
+// (rows inside the panel's columns) times the source's solved columns.
+struct SolveTask {
+  int64_t src_base;
+  int ld, k;
+  int rows_off;
+  int ca, cb;
+  int src_c0;  // first global column of the source's k columns
+};
+
+struct Plan {
+  int n = 0, nsuper = 0, npanel = 0;
+  int64_t xsize = 0;
+  std::vector<Panel> panels;
+  std::vector<Tile> tiles;          // grouped by level; diagonal tiles first per level
+  std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
+  std::vector<int> lvl_diag;        // per level: number of diagonal tiles
+  std::vector<Task> tasks;
+  std::vector<int> panel_level;     // per panel
+  std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order
+  std::vector<int> panel_order;     // panels grouped by level
+  std::vector<SolveTask> stasks;    // per panel, sources for the forward solve
+  std::vector<int> stask_ptr;       // npanel+1
+  double update_flops = 0, factor_flops = 0;
+  int nlevel = 0;
+};
+
+// super[nsuper+1], pi[nsuper+1] (row-list pointers), px[nsuper] value offsets,
+// Ls row indices, prunePtr/pruneSet as the reference's analysis produces them.
+void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
+                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P);
+
+}  // namespace swc

@@ -1,0 +1,113 @@
+"""Numpy emulation of the GPU kernels on the real device plan (test infrastructure).
+
+Runs the same index arithmetic as k_tile_update_factor / k_tile_trsm /
+k_fwd_panel / k_bwd_panel in swcholesky_amd/csrc/numeric.hip, asserting that every
+index stays inside its buffer, so plan and kernel indexing are checked on the CPU.
+"""
+import numpy as np
+
+
+def _chk(lo, hi, size, what):
+    assert 0 <= lo and hi <= size, f"{what}: [{lo},{hi}) outside [0,{size})"
+
+
+def factor(plan, Ls, xsize, amap, vals):
+    L = np.zeros(xsize)
+    _chk(int(amap.min()), int(amap.max()) + 1, xsize, "amap")
+    L[amap] = vals
+    panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
+    info = 0
+    for lvl in range(plan["nlevel"]):
+        t0, t1 = plan["lvl_tiles"][lvl], plan["lvl_tiles"][lvl + 1]
+        nd = plan["lvl_diag"][lvl]
+        for ti in range(t0, t1):
+            tl = tiles[ti]
+            pn = panels[tl["panel"]]
+            nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
+            assert 1 <= nr <= 64 and 1 <= nc <= 64
+            base = int(pn["vbase"]) + int(tl["r0"])
+            _chk(base, base + (nc - 1) * ld + nr, xsize, "tile")
+            ro = int(pn["rows_off"]) + int(tl["r0"])
+            _chk(ro, ro + nr, len(Ls), "tile rows")
+            trow = Ls[ro:ro + nr]
+            idx = base + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]
+            acc = L[idx].copy()
+            for t in range(tl["task_begin"], tl["task_end"]):
+                tk = tasks[t]
+                ra, rb, ca, cb = int(tk["ra"]), int(tk["rb"]), int(tk["ca"]), int(tk["cb"])
+                k, sld, sb = int(tk["k"]), int(tk["ld"]), int(tk["src_base"])
+                roff = int(tk["rows_off"])
+                assert ra < rb and ca < cb and rb - ra <= 64 and cb - ca <= 64
+                _chk(roff + min(ra, ca), roff + max(rb, cb), len(Ls), "task rows")
+                _chk(sb + min(ra, ca), sb + (k - 1) * sld + max(rb, cb), xsize, "task src")
+                rv = Ls[roff + ra:roff + rb]
+                cv = Ls[roff + ca:roff + cb]
+                pr = np.searchsorted(trow, rv)
+                assert np.all(pr < nr) and np.all(trow[pr] == rv), "row not in tile"
+                pc = cv - int(pn["c0"])
+                assert np.all((pc >= 0) & (pc < nc)), "column not in panel"
+                S = L[sb + np.arange(k)[None, :] * sld + np.arange(sld)[:, None]] if False else None
+                A = L[sb + np.arange(k)[None, :] * sld + np.arange(ra, rb)[:, None]]
+                B = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
+                C = A @ B.T
+                mask = trow[pr][:, None] >= (int(pn["c0"]) + pc)[None, :]
+                sub = acc[np.ix_(pr, pc)]
+                acc[np.ix_(pr, pc)] = np.where(mask, sub - C, sub)
+            if tl["kind"] == 0:
+                for j in range(nc):
+                    d = acc[j, j]
+                    if not d > 0:
+                        info = info or int(pn["c0"]) + j + 1
+                        d = 1.0
+                        acc[j, j] = 1.0
+                    else:
+                        acc[j, j] = np.sqrt(d)
+                    acc[j + 1:, j] /= acc[j, j]
+                    for c in range(j + 1, nc):
+                        acc[c:, c] -= acc[c:, j] * acc[c, j]
+                keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :]
+                L[idx[keep]] = acc[keep]
+            else:
+                L[idx] = acc
+        for ti in range(t0 + nd, t1):
+            tl = tiles[ti]
+            pn = panels[tl["panel"]]
+            nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
+            base = int(pn["vbase"]) + int(tl["r0"])
+            idx = base + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]
+            didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
+            l11 = np.tril(L[didx])
+            L[idx] = np.linalg.solve(l11, L[idx].T).T
+    return L, info
+
+
+def solve(plan, Ls, L, b):
+    """Forward then backward solve in the factor ordering (b is n-vector)."""
+    x = np.array(b, dtype=np.float64)
+    panels, st, stp, order = plan["panels"], plan["stasks"], plan["stask_ptr"], plan["panel_order"]
+    lp = plan["lvl_panels"]
+    for lvl in range(plan["nlevel"]):
+        for qi in range(lp[lvl], lp[lvl + 1]):
+            q = order[qi]
+            pn = panels[q]
+            nc, ld, c0 = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"])
+            t = x[c0:c0 + nc].copy()
+            for s in range(stp[q], stp[q + 1]):
+                tk = st[s]
+                ca, cb, k, sld, sb = (int(tk[f]) for f in ("ca", "cb", "k", "ld", "src_base"))
+                sc0, roff = int(tk["src_c0"]), int(tk["rows_off"])
+                A = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
+                pos = Ls[roff + ca:roff + cb] - c0
+                assert np.all((pos >= 0) & (pos < nc))
+                t[pos] -= A @ x[sc0:sc0 + k]
+            d = L[int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]]
+            x[c0:c0 + nc] = np.linalg.solve(np.tril(d), t)
+    for lvl in range(plan["nlevel"] - 1, -1, -1):
+        for qi in range(lp[lvl], lp[lvl + 1]):
+            pn = panels[order[qi]]
+            nc, ld, c0, nrow = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"]), int(pn["nrow"])
+            blk = L[int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nrow)[:, None]]
+            rows = Ls[int(pn["rows_off"]):int(pn["rows_off"]) + nrow]
+            t = x[c0:c0 + nc] - blk[nc:, :].T @ x[rows[nc:]]
+            x[c0:c0 + nc] = np.linalg.solve(np.tril(blk[:nc, :]).T, t)
+    return x

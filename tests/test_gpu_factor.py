@@ -1,0 +1,91 @@
+"""GPU parity: the HIP factorization and solves against the CPU oracle."""
+import numpy as np
+import pytest
+
+import swcholesky_amd as S
+import oracle_py as O
+from helpers import SMALL, full_csr, relF, setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,make", SMALL, ids=[s[0] for s in SMALL])
+def test_factor_matches_oracle(name, make):
+    prob = make()
+    b, a2x, Lo, info = setup(prob)
+    assert info == 0
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    Lg = f.L_values()
+    # FP64 L within 1e-12 relative Frobenius of the oracle factor
+    assert relF(Lg, Lo) < 1e-12, relF(Lg, Lo)
+
+
+@pytest.mark.parametrize("name,make", SMALL, ids=[s[0] for s in SMALL])
+def test_solve_backward_error(name, make):
+    prob = make()
+    b, a2x, Lo, info = setup(prob)
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    A, rp, ci, v = full_csr(prob)
+    rng = np.random.default_rng(0)
+    xs = rng.uniform(-1, 1, prob.n)
+    rhs = A @ xs
+    x = f.solve(rhs)
+    eta = O.backward_error(rp, ci, v, x, rhs)
+    assert eta < 1e-14, eta
+    xo = O.solve(b, Lo, rhs)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+
+
+def test_multi_rhs():
+    prob = SMALL[2][1]()
+    b, a2x, Lo, _ = setup(prob)
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    A, rp, ci, v = full_csr(prob)
+    B = np.random.default_rng(1).uniform(-1, 1, (prob.n, 8))
+    X = f.solve(B)
+    for j in range(8):
+        assert O.backward_error(rp, ci, v, X[:, j], B[:, j]) < 1e-14
+
+
+def test_legacy_entry_point_matches_oracle():
+    prob = SMALL[0][1]()
+    b, a2x, Lo, _ = setup(prob)
+    lv = np.zeros(b.xsize)
+    timing = np.zeros(8)
+    ok = S.sw_cholesky_left_par_05(b.n, b.a2p, b.a2i, a2x, b.p.astype(np.uint64), b.s,
+                                   b.i_ptr.astype(np.uint64), lv, b.super, b.nsuper, timing,
+                                   b.prunePtr, b.pruneSet)
+    assert ok == 1
+    assert relF(lv, Lo) < 1e-12
+    # strict upper part of every diagonal block stays zero
+    for s in range(b.nsuper):
+        c0, c1 = b.super[s], b.super[s + 1]
+        R = b.pi[s + 1] - b.pi[s]
+        blk = lv[b.p[c0]:b.p[c0] + R * (c1 - c0)].reshape(c1 - c0, R).T
+        assert np.all(np.triu(blk[:c1 - c0, :], 1) == 0)
+
+
+def test_not_positive_definite_reports_column():
+    from swcholesky_amd import problems as P
+    prob = P.laplacian2d_var(20, 5, shift=-2.0)  # strongly indefinite
+    b, a2x, Lo, info_o = setup(prob)
+    assert info_o > 0
+    f = S.Factor(b)
+    info = f.factor(prob.values)
+    assert info == info_o
+    with pytest.raises(S.SwcError):
+        f.solve(np.ones(prob.n))
+
+
+def test_refactor_same_structure_new_values():
+    prob = SMALL[3][1]()
+    b, a2x, Lo, _ = setup(prob)
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    v2 = prob.values * 2.0
+    assert f.factor(v2) == 0
+    Lo2, _ = O.factor(b, b.a2_values(v2))
+    assert relF(f.L_values(), Lo2) < 1e-12

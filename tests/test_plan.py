@@ -1,0 +1,61 @@
+"""The device plan and the kernels' index arithmetic, emulated on the CPU."""
+import numpy as np
+import pytest
+
+from swcholesky_amd import api, problems as P
+import emulate as E
+import oracle_py as O
+from helpers import relF, setup
+
+
+def _amap(b):
+    amap = np.zeros(len(b.a2i), np.int64)
+    for j in range(b.n):
+        s = b.col2Sup[j]
+        rows = b.s[b.pi[s]:b.pi[s + 1]]
+        q0, q1 = b.a2p[j], b.a2p[j + 1]
+        amap[q0:q1] = b.p[b.super[s]] + (j - b.super[s]) * len(rows) + np.searchsorted(
+            rows, b.a2i[q0:q1])
+    return amap
+
+
+@pytest.mark.parametrize("make", [lambda: P.laplacian2d_const(10), lambda: P.laplacian2d_var(37, 3),
+                                  lambda: P.laplacian3d_var(12), lambda: P.vector3d_var(6)])
+def test_emulated_kernels_match_oracle(make):
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    plan = api.build_plan(b)
+    Le, info = E.factor(plan, b.s, b.xsize, _amap(b), a2x)
+    assert info == 0
+    assert relF(Le, Lo) < 1e-12
+    y = np.random.default_rng(1).uniform(-1, 1, b.n)
+    xe = E.solve(plan, b.s, Le, y)
+    xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
+    assert relF(xe, xo) < 1e-6
+
+
+def test_plan_covers_every_update_once():
+    """Each (target entry, descendant) update appears in exactly one task."""
+    prob = P.laplacian3d_var(10)
+    b, *_ = setup(prob)
+    plan = api.build_plan(b)
+    panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
+    seen = {}
+    for tl in tiles:
+        pn = panels[tl["panel"]]
+        for t in tasks[tl["task_begin"]:tl["task_end"]]:
+            rows = b.s[t["rows_off"] + t["ra"]:t["rows_off"] + t["rb"]]
+            cols = b.s[t["rows_off"] + t["ca"]:t["rows_off"] + t["cb"]]
+            key = (int(t["src_base"]), int(t["k"]))
+            for r in rows:
+                for c in cols:
+                    if r >= c:
+                        seen.setdefault((int(r), int(c), key), 0)
+                        seen[(int(r), int(c), key)] += 1
+    assert all(v == 1 for v in seen.values())
+    # every level's tiles come after the tiles of the panels they read
+    lvl_of_panel = np.zeros(len(panels), int)
+    for l in range(plan["nlevel"]):
+        for tl in tiles[plan["lvl_tiles"][l]:plan["lvl_tiles"][l + 1]]:
+            lvl_of_panel[tl["panel"]] = l
+    assert np.all(np.diff(lvl_of_panel[plan["panel_order"]]) >= 0)

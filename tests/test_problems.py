@@ -1,0 +1,52 @@
+"""Generators, orderings and the reference's file formats."""
+import numpy as np
+import scipy.sparse.linalg as sla
+
+from swcholesky_amd import problems as P
+
+
+def test_generators_spd_and_sorted():
+    for prob in [P.laplacian2d_const(10), P.laplacian2d_var(9, 3), P.laplacian3d_var(5),
+                 P.vector3d_var(3), P.gmrf2d(8)]:
+        A = prob.scipy_full()
+        assert (A - A.T).count_nonzero() == 0
+        ev = np.linalg.eigvalsh(A.toarray())
+        assert ev.min() > 0
+        cp, ri = prob.colptr, prob.rowidx
+        for j in range(prob.n):
+            col = ri[cp[j]:cp[j + 1]]
+            assert col[0] == j and np.all(np.diff(col) > 0)
+        assert sorted(prob.perm) == list(range(prob.n))
+
+
+def test_config_sizes():
+    assert P.laplacian2d_const(100).nnz == 29800
+    c2 = P.laplacian3d_var(16)
+    assert c2.n == 16 ** 3 and c2.nnz == 16 ** 3 + 3 * 15 * 16 * 16
+    v = P.vector3d_var(4)
+    assert v.n == 3 * 64
+
+
+def test_nd_order_separators_last():
+    perm = P.grid_nd_order((7, 7))
+    assert perm[-1] in {21, 22, 23, 24, 25, 26, 27}  # middle row of a 7x7 grid last
+
+
+def test_mtx_and_perm_roundtrip(tmp_path):
+    prob = P.laplacian2d_var(6, 3)
+    f = tmp_path / "a.mtx"
+    P.write_mtx(str(f), prob.n, prob.colptr, prob.rowidx, prob.values)
+    n, cp, ri, va = P.read_mtx(str(f))
+    assert n == prob.n and np.array_equal(cp, prob.colptr) and np.array_equal(ri, prob.rowidx)
+    assert np.array_equal(va, prob.values)
+    pf = tmp_path / "a.perm"
+    P.write_perm(str(pf), prob.perm)
+    assert np.array_equal(P.read_perm(str(pf), prob.n), prob.perm)
+    assert pf.stat().st_size == 8 * prob.n
+
+
+def test_gmrf_precision_definition():
+    prob = P.gmrf2d(6, tau=2.0, kappa2=0.5)
+    A = prob.scipy_full()
+    lap = A / 2.0 - 0.5 * np.eye(prob.n)
+    assert np.allclose(np.asarray(lap.sum(axis=1)).ravel(), 0)
