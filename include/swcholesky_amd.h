@@ -111,6 +111,13 @@ int swc_get_factor(swc_handle* h, double* lValues, int on_device);
 int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device);
 /* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
 int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
+/* With profiling on, every kernel launch of swc_factor is bracketed by HIP events
+ * on the handle's stream (for measurement runs; adds launch gaps). */
+int swc_set_profiling(swc_handle* h, int on);
+/* out[8] = {last factor ms, last solve ms (device events), tile-kernel total ms,
+ * tile-kernel launches, trsm-kernel total ms, trsm launches (profiling only),
+ * tile-kernel algorithmic flops, trsm-kernel algorithmic flops}. */
+int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the

@@ -37,6 +37,8 @@ SIGNATURES = {
     "swc_get_factor": (c_int, [c_vp, c_vp, c_int]),
     "swc_solve": (c_int, [c_vp, c_vp, c_int, c_i64, c_int]),
     "swc_plan_stats": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
+    "swc_set_profiling": (c_int, [c_vp, c_int]),
+    "swc_get_timings": (c_int, [c_vp, c_vp]),
     "swc_plan_build": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                ctypes.POINTER(c_vp)]),
     "swc_plan_free": (None, [c_vp]),

@@ -153,6 +153,16 @@ class Factor:
         check(_lib.lib().swc_solve(self._h, p, nrhs, self.n, dev))
         return B
 
+    def set_profiling(self, on=True):
+        check(_lib.lib().swc_set_profiling(self._h, int(on)))
+
+    def timings(self):
+        out = np.zeros(8)
+        check(_lib.lib().swc_get_timings(self._h, ptr(out)))
+        keys = ["factor_ms", "solve_ms", "tile_ms", "tile_launches", "trsm_ms", "trsm_launches",
+                "tile_flops", "trsm_flops"]
+        return dict(zip(keys, out.tolist()))
+
     def plan_stats(self):
         out = np.zeros(5, dtype=np.int64)
         fl = (ctypes.c_double * 2)()

@@ -58,6 +58,8 @@ __global__ void k_assemble(double* __restrict__ L, const int64_t* __restrict__ a
   for (; q < nnz; q += stride) L[amap[q]] = vals[q];
 }
 
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
 // position of value v in the sorted list a[0..len) (v is known to be present)
 __device__ inline int find_pos(const int* a, int len, int v) {
   int lo = 0, hi = len;
@@ -67,6 +69,47 @@ __device__ inline int find_pos(const int* a, int len, int v) {
     else hi = mid;
   }
   return lo;
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// One task's product C = A B^T, A = source rows [ra, ra+M), B = source rows
+// [ca, ca+N), inner dimension k (source columns, stride ld), on FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  Wave w owns the 32x32 quadrant (w/2, w%2) of the
+// 64x64 output as 2x2 blocks of 16x16; lane l feeds A[l%16][k0+l/16] and
+// B^T[k0+l/16][l%16] and holds D[l/16 + 4r][l%16] in component r (rows of the
+// f64 variant are interleaved by 4, unlike the f32 variant's D[4(l/16)+r]; an
+// on-device probe of both builtins fixed these layouts).
+__device__ inline void task_mfma(const double* __restrict__ src, int ld, int k, int ra, int M,
+                                 int ca, int N, int wave, int lane, d4 (&c)[2][2]) {
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+  if (wr >= M || wc >= N) return;  // wave-uniform: quadrant outside the task block
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = wr + lr, r1 = wr + 16 + lr;
+  const int q0 = wc + lr, q1 = wc + 16 + lr;
+  const bool a0ok = r0 < M, a1ok = r1 < M, b0ok = q0 < N, b1ok = q1 < N;
+  const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;  // wave-uniform
+  const double* pa = src + ra;
+  const double* pb = src + ca;
+  for (int kk = 0; kk < k; kk += 4) {
+    const int kc = kk + lk;
+    const bool kok = kc < k;
+    const int64_t off = (int64_t)kc * ld;
+    const double a0 = (a0ok && kok) ? pa[off + r0] : 0.0;
+    const double a1 = (a1ok && kok) ? pa[off + r1] : 0.0;
+    const double b0 = (b0ok && kok) ? pb[off + q0] : 0.0;
+    const double b1 = (b1ok && kok) ? pb[off + q1] : 0.0;
+    c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0][0], 0, 0, 0);
+    if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[0][1], 0, 0, 0);
+    if (blk_a1) {
+      c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[1][0], 0, 0, 0);
+      if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[1][1], 0, 0, 0);
+    }
+  }
 }
 
 // Tile kernel: accumulate all source updates of one tile in LDS (in task order),
@@ -84,7 +127,7 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
   __shared__ int bad;
   const Tile tl = tiles[tile0 + blockIdx.x];
   const Panel pn = panels[tl.panel];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = tl.nr, nc = pn.ncol;
   double* gtile = L + pn.vbase + tl.r0;
   for (int e = tid; e < nr; e += kThreads) trow[e] = Ls[pn.rows_off + tl.r0 + e];
@@ -101,15 +144,35 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
     for (int e = tid; e < N; e += kThreads) pc[e] = Ls[tk.rows_off + tk.ca + e] - pn.c0;
     __syncthreads();
     const double* src = L + tk.src_base;
-    for (int e = tid; e < M * N; e += kThreads) {
-      const int i = e % M, j = e / M;
-      const int ti = pr[i], tj = pc[j];
-      if (trow[ti] < pn.c0 + tj) continue;  // strict upper part of the diagonal block
-      const double* a = src + tk.ra + i;
-      const double* b = src + tk.ca + j;
-      double s = 0.0;
-      for (int kk = 0; kk < tk.k; ++kk) s = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], s);
-      acc[tj * kLdsLd + ti] -= s;
+    if (tk.k < 8) {
+      // short inner dimension: one thread per output element
+      for (int e = tid; e < M * N; e += kThreads) {
+        const int i = e % M, j = e / M;
+        const int ti = pr[i], tj = pc[j];
+        if (trow[ti] < pn.c0 + tj) continue;  // strict upper part of the diagonal block
+        const double* a = src + tk.ra + i;
+        const double* b = src + tk.ca + j;
+        double s = 0.0;
+        for (int kk = 0; kk < tk.k; ++kk) s = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], s);
+        acc[tj * kLdsLd + ti] -= s;
+      }
+    } else {
+      d4 c[2][2];
+      task_mfma(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
+      const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = wr + 16 * bi + (lane >> 4) + 4 * r;
+            const int j = wc + 16 * bj + (lane & 15);
+            if (i < M && j < N) {
+              const int ti = pr[i], tj = pc[j];
+              if (trow[ti] >= pn.c0 + tj) acc[tj * kLdsLd + ti] -= c[bi][bj][r];
+            }
+          }
     }
     __syncthreads();
   }
@@ -307,6 +370,12 @@ struct swc_handle {
   int* dInfo = nullptr;
   bool factored = false;
   int last_info = 0;
+  // timing (device events on `stream`)
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool profile = false;
+  std::vector<hipEvent_t> pev;  // per-launch pairs when profiling
+  double t_factor_ms = 0, t_solve_ms = 0, t_tile_ms = 0, t_trsm_ms = 0;
+  int n_tile = 0, n_trsm = 0;
 
   ~swc_handle() { release(); }
   void release() {
@@ -316,6 +385,10 @@ struct swc_handle {
     dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
     dSt = nullptr; dStp = nullptr; dOrder = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    for (auto& e : pev) hipEventDestroy(e);
+    pev.clear();
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -380,25 +453,57 @@ int ensure_work(swc_handle* h, int cols) {
 int factor_device(swc_handle* h, const double* dvals) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
+  if (h->profile && h->pev.size() < (size_t)4 * P.nlevel) {
+    for (auto& e : h->pev) hipEventDestroy(e);
+    h->pev.assign((size_t)4 * P.nlevel, nullptr);
+    for (auto& e : h->pev) SWC_HIP(hipEventCreate(&e));
+  }
+  SWC_HIP(hipEventRecord(h->ev[0], st));
   hipLaunchKernelGGL(k_fill, dim3(grid_for(P.xsize)), dim3(kThreads), 0, st, h->dL, P.xsize, 0.0);
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(h->nvals)), dim3(kThreads), 0, st, h->dL,
                      (const int64_t*)h->dAmap, dvals, h->nvals);
   const int big = 0x7fffffff;
-  SWC_HIP(hipMemcpyAsync(h->dInfo, &big, sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, big);
   for (int l = 0; l < P.nlevel; ++l) {
     const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0, nd = P.lvl_diag[l];
+    hipEvent_t* pe = h->profile ? &h->pev[(size_t)4 * l] : nullptr;
+    if (pe) SWC_HIP(hipEventRecord(pe[0], st));
     if (nt > 0)
       hipLaunchKernelGGL(k_tile_update_factor, dim3(nt), dim3(kThreads), 0, st, h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
                          (const Task*)h->dTasks, t0, h->dInfo);
+    if (pe) SWC_HIP(hipEventRecord(pe[1], st));
     if (nt - nd > 0)
       hipLaunchKernelGGL(k_tile_trsm, dim3(nt - nd), dim3(kThreads), 0, st, h->dL,
                          (const Panel*)h->dPanels, (const Tile*)h->dTiles, t0 + nd);
+    if (pe) SWC_HIP(hipEventRecord(pe[2], st));
   }
+  SWC_HIP(hipEventRecord(h->ev[1], st));
   SWC_HIP(hipGetLastError());
   int info = 0;
   SWC_HIP(hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st));
   SWC_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  SWC_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  h->t_factor_ms = ms;
+  if (h->profile) {
+    h->t_tile_ms = h->t_trsm_ms = 0;
+    h->n_tile = h->n_trsm = 0;
+    for (int l = 0; l < P.nlevel; ++l) {
+      const int nt = P.lvl_tiles[l + 1] - P.lvl_tiles[l], nd = P.lvl_diag[l];
+      hipEvent_t* pe = &h->pev[(size_t)4 * l];
+      if (nt > 0) {
+        SWC_HIP(hipEventElapsedTime(&ms, pe[0], pe[1]));
+        h->t_tile_ms += ms;
+        h->n_tile++;
+      }
+      if (nt - nd > 0) {
+        SWC_HIP(hipEventElapsedTime(&ms, pe[1], pe[2]));
+        h->t_trsm_ms += ms;
+        h->n_trsm++;
+      }
+    }
+  }
   h->last_info = info == big ? 0 : info;
   h->factored = true;
   return SWC_OK;
@@ -408,6 +513,7 @@ int factor_device(swc_handle* h, const double* dvals) {
 int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
+  SWC_HIP(hipEventRecord(h->ev[2], st));
   for (int l = 0; l < P.nlevel; ++l) {
     const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
     if (np > 0)
@@ -422,6 +528,7 @@ int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
                          dX, nrhs, ldx);
   }
+  SWC_HIP(hipEventRecord(h->ev[3], st));
   SWC_HIP(hipGetLastError());
   return SWC_OK;
 }
@@ -443,6 +550,8 @@ int swc_create(swc_handle** out) {
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&h->dInfo, sizeof(int));
+  for (auto& ev : h->ev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
   if (e != hipSuccess) {
     delete h;
     return swc_set_error(SWC_EDEVICE, std::string("swc_create: ") + hipGetErrorString(e));
@@ -629,6 +738,30 @@ int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device) {
       }
   }
   SWC_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  SWC_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+  h->t_solve_ms = ms;
+  return SWC_OK;
+}
+
+int swc_set_profiling(swc_handle* h, int on) {
+  if (int rc = check_handle(h)) return rc;
+  h->profile = on != 0;
+  return SWC_OK;
+}
+
+int swc_get_timings(const swc_handle* h, double* out) {
+  if (int rc = check_handle(h)) return rc;
+  if (!out) return swc_set_error(SWC_EINVAL, "swc_get_timings: null argument");
+  const Plan& P = h->plan;
+  out[0] = h->t_factor_ms;
+  out[1] = h->t_solve_ms;
+  out[2] = h->t_tile_ms;
+  out[3] = h->n_tile;
+  out[4] = h->t_trsm_ms;
+  out[5] = h->n_trsm;
+  out[6] = P.tile_kernel_flops;
+  out[7] = P.trsm_kernel_flops;
   return SWC_OK;
 }
 

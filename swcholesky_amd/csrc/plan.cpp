@@ -171,6 +171,13 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
             t.rb = rb;
             P.tasks.push_back(t);
             P.update_flops += 2.0 * (rb - ra) * (double)(t.cb - t.ca) * t.k;
+            // useful products: pairs (row, col) with row >= col (both sorted)
+            double pairs = 0;
+            for (int c = t.ca, r = ra; c < t.cb; ++c) {
+              while (r < rb && rows[r] < rows[c]) ++r;
+              pairs += rb - r;
+            }
+            P.tile_kernel_flops += 2.0 * pairs * t.k;
           }
           tl.task_end = (int)P.tasks.size();
           P.tiles.push_back(tl);
@@ -183,6 +190,9 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   for (const Panel& pn : P.panels) {
     const double w = pn.ncol, m = pn.nrow - pn.ncol;
     P.factor_flops += w * w * w / 3.0 + m * w * w;
+    const double m0 = std::min(pn.nrow, TM) - w;  // rows solved inside the diagonal tile
+    P.tile_kernel_flops += w * w * w / 3.0 + m0 * w * w;
+    P.trsm_kernel_flops += (m - m0) * w * w;
   }
 }
 

@@ -72,6 +72,9 @@ struct Plan {
   std::vector<SolveTask> stasks;    // per panel, sources for the forward solve
   std::vector<int> stask_ptr;       // npanel+1
   double update_flops = 0, factor_flops = 0;
+  // algorithmic flops of each kernel class: lower-triangle update products and
+  // the diagonal-tile POTRF + in-tile TRSM (tile kernel), off-diagonal TRSM
+  double tile_kernel_flops = 0, trsm_kernel_flops = 0;
   int nlevel = 0;
 };
 
