@@ -121,11 +121,12 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[6] = {panels, tiles, tasks, levels,
- * solve tasks, xsize}; swc_plan_export copies array `which` (0 panels, 1 tiles,
- * 2 tasks, 3 level->tile offsets, 4 diagonal tiles per level, 5 panels by level,
- * 6 level->panel offsets, 7 solve tasks, 8 panel->solve-task offsets) as the
- * packed structs of swcholesky_amd/csrc/plan.h. */
+ * device, for tools and tests.  sizes[7] = {panels, tiles, tasks, levels,
+ * solve tasks, xsize, TRSM tiles}; swc_plan_export copies array `which`
+ * (0 panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->TRSM-list
+ * offsets, 5 panels by level, 6 level->panel offsets, 7 solve tasks,
+ * 8 panel->solve-task offsets, 9 TRSM tile list) as the packed structs of
+ * swcholesky_amd/csrc/plan.h. */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,

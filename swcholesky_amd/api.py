@@ -196,15 +196,16 @@ def build_plan(bcsc: BCSC):
     check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
                              ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
     try:
-        sz = np.zeros(6, dtype=np.int64)
+        sz = np.zeros(7, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, nstask, xsize = (int(v) for v in sz)
+        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm = (int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
-                (3, np.int32, nlevel + 1), (4, np.int32, nlevel), (5, np.int32, npanel),
-                (6, np.int32, nlevel + 1), (7, STASK_DT, nstask), (8, np.int32, npanel + 1)]
-        names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_diag", "panel_order",
-                 "lvl_panels", "stasks", "stask_ptr"]
+                (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
+                (6, np.int32, nlevel + 1), (7, STASK_DT, nstask), (8, np.int32, npanel + 1),
+                (9, np.int32, ntrsm)]
+        names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
+                 "lvl_panels", "stasks", "stask_ptr", "trsm_tiles"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)

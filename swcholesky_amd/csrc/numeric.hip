@@ -211,10 +211,10 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
 // Off-diagonal tile TRSM: B <- B L11^{-T}, L11 the panel's factored diagonal block.
 __global__ void __launch_bounds__(kThreads)
 k_tile_trsm(double* __restrict__ L, const Panel* __restrict__ panels,
-            const Tile* __restrict__ tiles, int tile0) {
+            const Tile* __restrict__ tiles, const int* __restrict__ list, int list0) {
   __shared__ double acc[swc::PW * kLdsLd];
   __shared__ double l11[swc::PW * kLdsLd];
-  const Tile tl = tiles[tile0 + blockIdx.x];
+  const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
   const int tid = threadIdx.x;
   const int nr = tl.nr, nc = pn.ncol;
@@ -362,6 +362,7 @@ struct swc_handle {
   SolveTask* dSt = nullptr;
   int* dStp = nullptr;
   int* dOrder = nullptr;
+  int* dTrsm = nullptr;
   int64_t* dAmap = nullptr;
   int* dPerm = nullptr;
   double* dVals = nullptr;  // staging for host values
@@ -380,10 +381,10 @@ struct swc_handle {
   ~swc_handle() { release(); }
   void release() {
     hipFree(dL); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
-    hipFree(dSt); hipFree(dStp); hipFree(dOrder); hipFree(dAmap); hipFree(dPerm);
+    hipFree(dSt); hipFree(dStp); hipFree(dOrder); hipFree(dTrsm); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
     dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dSt = nullptr; dStp = nullptr; dOrder = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dSt = nullptr; dStp = nullptr; dOrder = nullptr; dTrsm = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
@@ -465,7 +466,8 @@ int factor_device(swc_handle* h, const double* dvals) {
   const int big = 0x7fffffff;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, big);
   for (int l = 0; l < P.nlevel; ++l) {
-    const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0, nd = P.lvl_diag[l];
+    const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0;
+    const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
     hipEvent_t* pe = h->profile ? &h->pev[(size_t)4 * l] : nullptr;
     if (pe) SWC_HIP(hipEventRecord(pe[0], st));
     if (nt > 0)
@@ -473,9 +475,10 @@ int factor_device(swc_handle* h, const double* dvals) {
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
                          (const Task*)h->dTasks, t0, h->dInfo);
     if (pe) SWC_HIP(hipEventRecord(pe[1], st));
-    if (nt - nd > 0)
-      hipLaunchKernelGGL(k_tile_trsm, dim3(nt - nd), dim3(kThreads), 0, st, h->dL,
-                         (const Panel*)h->dPanels, (const Tile*)h->dTiles, t0 + nd);
+    if (ns > 0)
+      hipLaunchKernelGGL(k_tile_trsm, dim3(ns), dim3(kThreads), 0, st, h->dL,
+                         (const Panel*)h->dPanels, (const Tile*)h->dTiles,
+                         (const int*)h->dTrsm, s0);
     if (pe) SWC_HIP(hipEventRecord(pe[2], st));
   }
   SWC_HIP(hipEventRecord(h->ev[1], st));
@@ -490,14 +493,15 @@ int factor_device(swc_handle* h, const double* dvals) {
     h->t_tile_ms = h->t_trsm_ms = 0;
     h->n_tile = h->n_trsm = 0;
     for (int l = 0; l < P.nlevel; ++l) {
-      const int nt = P.lvl_tiles[l + 1] - P.lvl_tiles[l], nd = P.lvl_diag[l];
+      const int nt = P.lvl_tiles[l + 1] - P.lvl_tiles[l];
+      const int ns = P.lvl_trsm[l + 1] - P.lvl_trsm[l];
       hipEvent_t* pe = &h->pev[(size_t)4 * l];
       if (nt > 0) {
         SWC_HIP(hipEventElapsedTime(&ms, pe[0], pe[1]));
         h->t_tile_ms += ms;
         h->n_tile++;
       }
-      if (nt - nd > 0) {
+      if (ns > 0) {
         SWC_HIP(hipEventElapsedTime(&ms, pe[1], pe[2]));
         h->t_trsm_ms += ms;
         h->n_trsm++;
@@ -609,6 +613,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dSt, P.stasks.data(), P.stasks.size(), st);
   if (!rc) rc = upload(&h->dStp, P.stask_ptr.data(), P.stask_ptr.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
+  if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
   if (!rc) rc = upload(&h->dAmap, amap.data(), amap.size(), st);
   if (!rc) rc = upload(&h->dVals, (const double*)nullptr, (size_t)h->nvals, st);
   if (rc) return rc;

@@ -47,11 +47,9 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   first_panel[nsuper] = (int)P.panels.size();
   P.npanel = (int)P.panels.size();
 
-  // ---- sources of each panel, and levels ----
+  // ---- descendant sources of each panel ----
   struct Src { int d; int ca, cb; };  // d >= 0: supernode, d < 0: panel -(d+1) of same sn
-  std::vector<std::vector<Src>> srcs(P.npanel);
-  P.panel_level.assign(P.npanel, 0);
-  std::vector<int> sn_level(nsuper, -1);  // level of the last panel of each supernode
+  std::vector<std::vector<Src>> srcs(P.npanel);  // descendants touching each panel
   for (int s = 0; s < nsuper; ++s) {
     const int fp = first_panel[s], lp = first_panel[s + 1];
     for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) {
@@ -63,35 +61,11 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
         const Panel& pn = P.panels[q];
         const int ca = lower_idx(rows, len, pn.c0);
         const int cb = lower_idx(rows, len, pn.c0 + pn.ncol);
-        if (ca < cb) {
-          srcs[q].push_back({d, ca, cb});
-          P.panel_level[q] = std::max(P.panel_level[q], sn_level[d] + 1);
-        }
+        if (ca < cb) srcs[q].push_back({d, ca, cb});
       }
     }
-    for (int q = fp; q < lp; ++q) {
-      for (int q2 = fp; q2 < q; ++q2) {
-        const int off = (q - fp) * PW;
-        srcs[q].push_back({-(q2 + 1), off, off + P.panels[q].ncol});
-      }
-      if (q > fp) P.panel_level[q] = std::max(P.panel_level[q], P.panel_level[q - 1] + 1);
-    }
-    sn_level[s] = P.panel_level[lp - 1];
-  }
-  P.nlevel = 0;
-  for (int q = 0; q < P.npanel; ++q) P.nlevel = std::max(P.nlevel, P.panel_level[q] + 1);
-
-  // ---- panels grouped by level ----
-  P.lvl_panels.assign(P.nlevel + 1, 0);
-  for (int q = 0; q < P.npanel; ++q) P.lvl_panels[P.panel_level[q] + 1]++;
-  for (int l = 0; l < P.nlevel; ++l) P.lvl_panels[l + 1] += P.lvl_panels[l];
-  P.panel_order.resize(P.npanel);
-  {
-    std::vector<int> fill(P.lvl_panels.begin(), P.lvl_panels.end() - 1);
-    for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
   }
 
-  // ---- source descriptors: task prototypes shared by the tiles of a panel ----
   auto describe = [&](int q, const Src& sr, Task& t, SolveTask& st) {
     const Panel& pn = P.panels[q];
     if (sr.d >= 0) {
@@ -122,70 +96,162 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     st.cb = sr.cb;
   };
 
-  // forward-solve sources
+  // ---- forward-solve sources: descendants, then earlier panels of the supernode ----
   P.stask_ptr.assign(P.npanel + 1, 0);
-  for (int q = 0; q < P.npanel; ++q) {
+  for (int s = 0; s < nsuper; ++s) {
+    const int fp = first_panel[s], lp = first_panel[s + 1];
+    for (int q = fp; q < lp; ++q) {
+      Task t;
+      SolveTask st;
+      for (const Src& sr : srcs[q]) {
+        describe(q, sr, t, st);
+        P.stasks.push_back(st);
+      }
+      for (int q2 = fp; q2 < q; ++q2) {
+        const int off = (q - fp) * PW;
+        describe(q, {-(q2 + 1), off, off + P.panels[q].ncol}, t, st);
+        P.stasks.push_back(st);
+      }
+      P.stask_ptr[q + 1] = (int)P.stasks.size();
+    }
+  }
+
+  // ---- levels: tiles of each level form one update launch, followed by one
+  // TRSM launch over the tiles whose panel diagonal that level factored ----
+  struct PTile { Tile t; int level; bool trsm; };
+  std::vector<PTile> ptiles;
+  std::vector<std::vector<Task>> ptasks;
+  P.panel_level.assign(P.npanel, 0);
+  std::vector<int> sn_level(nsuper, -1);  // level at which a supernode is complete
+  auto add_tile = [&](int q, int r0, int kind, int level, bool trsm) {
+    const Panel& pn = P.panels[q];
+    PTile pt;
+    pt.t.panel = q;
+    pt.t.r0 = r0;
+    pt.t.nr = std::min(TM, pn.nrow - r0);
+    pt.t.kind = kind;
+    pt.t.pad0 = pt.t.pad1 = 0;
+    pt.t.task_begin = pt.t.task_end = 0;
+    pt.level = level;
+    pt.trsm = trsm;
+    ptiles.push_back(pt);
+    ptasks.emplace_back();
+    return (int)ptiles.size() - 1;
+  };
+  // descendant tasks of tile (panel q, rows r0..r0+nr) in pruneSet order
+  auto add_desc_tasks = [&](int ti) {
+    const Tile& tl = ptiles[ti].t;
+    const int q = tl.panel;
+    const Panel& pn = P.panels[q];
+    const int vfirst = Ls[pn.rows_off + tl.r0];
+    const int vlast = Ls[pn.rows_off + tl.r0 + tl.nr - 1];
     for (const Src& sr : srcs[q]) {
       Task t;
       SolveTask st;
       describe(q, sr, t, st);
-      P.stasks.push_back(st);
+      const int* rows = Ls + t.rows_off;
+      const int len = (int)(pi[sr.d + 1] - pi[sr.d]);
+      const int ra = std::max(lower_idx(rows, len, vfirst), sr.ca);
+      const int rb = upper_idx(rows, len, vlast);
+      if (ra >= rb) continue;
+      t.ra = ra;
+      t.rb = rb;
+      ptasks[ti].push_back(t);
     }
-    P.stask_ptr[q + 1] = (int)P.stasks.size();
-  }
+  };
+  // task of tile (panel q, rows r0..) from the earlier panel q2 of the same supernode
+  auto add_panel_task = [&](int ti, int q2) {
+    const Tile& tl = ptiles[ti].t;
+    const int q = tl.panel;
+    const Panel& pn = P.panels[q];
+    const int off = pn.c0 - super[pn.sn];
+    Task t;
+    SolveTask st;
+    describe(q, {-(q2 + 1), off, off + pn.ncol}, t, st);
+    t.ra = off + tl.r0;
+    t.rb = t.ra + tl.nr;
+    ptasks[ti].push_back(t);
+  };
 
-  // ---- tiles and tasks, level by level (diagonal tiles first in a level) ----
-  P.lvl_tiles.assign(P.nlevel + 1, 0);
-  P.lvl_diag.assign(P.nlevel, 0);
-  for (int l = 0; l < P.nlevel; ++l) {
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int qi = P.lvl_panels[l]; qi < P.lvl_panels[l + 1]; ++qi) {
-        const int q = P.panel_order[qi];
-        const Panel& pn = P.panels[q];
-        for (int r0 = pass == 0 ? 0 : TM; r0 < (pass == 0 ? 1 : pn.nrow); r0 += TM) {
-          Tile tl;
-          tl.panel = q;
-          tl.r0 = r0;
-          tl.nr = std::min(TM, pn.nrow - r0);
-          tl.kind = pass;
-          tl.pad0 = tl.pad1 = 0;
-          tl.task_begin = (int)P.tasks.size();
-          const int vfirst = Ls[pn.rows_off + r0];
-          const int vlast = Ls[pn.rows_off + r0 + tl.nr - 1];
-          for (const Src& sr : srcs[q]) {
-            Task t;
-            SolveTask st;
-            describe(q, sr, t, st);
-            const int* rows = Ls + t.rows_off;
-            int ra, rb;
-            if (sr.d >= 0) {
-              const int len = (int)(pi[sr.d + 1] - pi[sr.d]);
-              ra = std::max(lower_idx(rows, len, vfirst), sr.ca);
-              rb = upper_idx(rows, len, vlast);
-            } else {
-              ra = (pn.c0 - super[pn.sn]) + r0;
-              rb = ra + tl.nr;
-            }
-            if (ra >= rb) continue;
-            t.ra = ra;
-            t.rb = rb;
-            P.tasks.push_back(t);
-            P.update_flops += 2.0 * (rb - ra) * (double)(t.cb - t.ca) * t.k;
-            // useful products: pairs (row, col) with row >= col (both sorted)
-            double pairs = 0;
-            for (int c = t.ca, r = ra; c < t.cb; ++c) {
-              while (r < rb && rows[r] < rows[c]) ++r;
-              pairs += rb - r;
-            }
-            P.tile_kernel_flops += 2.0 * pairs * t.k;
-          }
-          tl.task_end = (int)P.tasks.size();
-          P.tiles.push_back(tl);
-          if (pass == 0) P.lvl_diag[l]++;
-        }
+  for (int s = 0; s < nsuper; ++s) {
+    const int fp = first_panel[s], lp = first_panel[s + 1], np = lp - fp;
+    int le = 0;
+    for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) le = std::max(le, sn_level[pruneSet[t]] + 1);
+    // external phase (and, for one panel, the whole factorization) at level le
+    for (int J = 0; J < np; ++J) {
+      const Panel& pn = P.panels[fp + J];
+      for (int r0 = 0; r0 < pn.nrow; r0 += TM) {
+        const bool diag = r0 == 0;
+        const int kind = (diag && J == 0) ? 0 : 1;
+        const int ti = add_tile(fp + J, r0, kind, le, !diag && J == 0);
+        add_desc_tasks(ti);
       }
     }
-    P.lvl_tiles[l + 1] = (int)P.tiles.size();
+    P.panel_level[fp] = le;
+    // right-looking steps: step p at level le+p+1 updates panels J > p from panel p
+    for (int p = 0; p + 1 < np; ++p) {
+      const int lvl = le + p + 1;
+      for (int J = p + 1; J < np; ++J) {
+        const Panel& pn = P.panels[fp + J];
+        for (int r0 = 0; r0 < pn.nrow; r0 += TM) {
+          const bool diag = r0 == 0;
+          const int kind = (diag && J == p + 1) ? 0 : 1;
+          const int ti = add_tile(fp + J, r0, kind, lvl, !diag && J == p + 1);
+          add_panel_task(ti, fp + p);
+        }
+      }
+      P.panel_level[fp + p + 1] = lvl;
+    }
+    sn_level[s] = le + np - 1;
+  }
+  P.nlevel = 0;
+  for (const PTile& pt : ptiles) P.nlevel = std::max(P.nlevel, pt.level + 1);
+
+  // ---- panels grouped by level (forward solve order) ----
+  P.lvl_panels.assign(P.nlevel + 1, 0);
+  for (int q = 0; q < P.npanel; ++q) P.lvl_panels[P.panel_level[q] + 1]++;
+  for (int l = 0; l < P.nlevel; ++l) P.lvl_panels[l + 1] += P.lvl_panels[l];
+  P.panel_order.resize(P.npanel);
+  {
+    std::vector<int> fill(P.lvl_panels.begin(), P.lvl_panels.end() - 1);
+    for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
+  }
+
+  // ---- flatten: tiles by level (stable), tasks, TRSM lists ----
+  std::vector<int> order(ptiles.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return ptiles[a].level < ptiles[b].level; });
+  P.lvl_tiles.assign(P.nlevel + 1, 0);
+  P.lvl_trsm.assign(P.nlevel + 1, 0);
+  std::vector<std::vector<int>> trsm_by_level(P.nlevel);
+  for (int oi : order) {
+    PTile& pt = ptiles[oi];
+    pt.t.task_begin = (int)P.tasks.size();
+    const Panel& pn = P.panels[pt.t.panel];
+    const int* trow = Ls + pn.rows_off + pt.t.r0;
+    for (const Task& t : ptasks[oi]) {
+      P.tasks.push_back(t);
+      P.update_flops += 2.0 * (t.rb - t.ra) * (double)(t.cb - t.ca) * t.k;
+      // useful products: pairs (row, col) with row >= col (both sorted)
+      const int* rows = Ls + t.rows_off;
+      double pairs = 0;
+      for (int c = t.ca, r = t.ra; c < t.cb; ++c) {
+        while (r < t.rb && rows[r] < rows[c]) ++r;
+        pairs += t.rb - r;
+      }
+      P.tile_kernel_flops += 2.0 * pairs * t.k;
+    }
+    (void)trow;
+    pt.t.task_end = (int)P.tasks.size();
+    P.lvl_tiles[pt.level + 1]++;
+    if (pt.trsm) trsm_by_level[pt.level].push_back((int)P.tiles.size());
+    P.tiles.push_back(pt.t);
+  }
+  for (int l = 0; l < P.nlevel; ++l) P.lvl_tiles[l + 1] += P.lvl_tiles[l];
+  for (int l = 0; l < P.nlevel; ++l) {
+    P.trsm_tiles.insert(P.trsm_tiles.end(), trsm_by_level[l].begin(), trsm_by_level[l].end());
+    P.lvl_trsm[l + 1] = (int)P.trsm_tiles.size();
   }
   for (const Panel& pn : P.panels) {
     const double w = pn.ncol, m = pn.nrow - pn.ncol;
@@ -237,6 +303,7 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[3] = P.nlevel;
   out[4] = (int64_t)P.stasks.size();
   out[5] = P.xsize;
+  out[6] = (int64_t)P.trsm_tiles.size();
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
@@ -256,11 +323,12 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 1: return cp(P.tiles);
     case 2: return cp(P.tasks);
     case 3: return cp(P.lvl_tiles);
-    case 4: return cp(P.lvl_diag);
+    case 4: return cp(P.lvl_trsm);
     case 5: return cp(P.panel_order);
     case 6: return cp(P.lvl_panels);
     case 7: return cp(P.stasks);
     case 8: return cp(P.stask_ptr);
+    case 9: return cp(P.trsm_tiles);
     default: return swc_set_error(SWC_EINVAL, "swc_plan_export: unknown array id");
   }
 }

@@ -1,12 +1,16 @@
 // Device work plan of the numeric phase, built once per symbolic structure.
 //
-// Every supernode is cut into panels of at most PW consecutive columns; a panel
-// is a target of the left-looking factorization.  The sources of a panel are the
-// reference's pruned descendants of its supernode that have rows in the panel's
-// columns (in pruneSet order) followed by the earlier panels of the same
-// supernode.  Each panel's rows are cut into tiles of TM rows; a tile owns its
-// accumulator, so every entry of L is updated by exactly one workgroup, in a
-// fixed source order, without atomics.
+// Every supernode is cut into panels of at most PW consecutive columns and its
+// rows into tiles of TM rows; a tile owns its accumulator, so each entry of L is
+// updated by one workgroup at a time, in a fixed source order, without atomics.
+//
+// A supernode of one panel is one left-looking target: its tiles apply the
+// updates of the reference's pruned descendants (pruneSet order), the diagonal
+// tile then factors the panel, and the other tiles are solved against it.
+// A wider supernode first applies all descendant updates to every tile of its
+// lower trapezoid in a single level (fully 2D-parallel), then factors itself
+// right-looking: step p factors panel p and updates the trailing tiles from it,
+// one level per step.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -62,9 +66,10 @@ struct Plan {
   int n = 0, nsuper = 0, npanel = 0;
   int64_t xsize = 0;
   std::vector<Panel> panels;
-  std::vector<Tile> tiles;          // grouped by level; diagonal tiles first per level
+  std::vector<Tile> tiles;          // grouped by level (one update launch per level)
   std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
-  std::vector<int> lvl_diag;        // per level: number of diagonal tiles
+  std::vector<int> trsm_tiles;      // tiles solved against their panel's diagonal block
+  std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles
   std::vector<Task> tasks;
   std::vector<int> panel_level;     // per panel
   std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order

@@ -19,7 +19,7 @@ def factor(plan, Ls, xsize, amap, vals):
     info = 0
     for lvl in range(plan["nlevel"]):
         t0, t1 = plan["lvl_tiles"][lvl], plan["lvl_tiles"][lvl + 1]
-        nd = plan["lvl_diag"][lvl]
+        s0, s1 = plan["lvl_trsm"][lvl], plan["lvl_trsm"][lvl + 1]
         for ti in range(t0, t1):
             tl = tiles[ti]
             pn = panels[tl["panel"]]
@@ -69,7 +69,7 @@ def factor(plan, Ls, xsize, amap, vals):
                 L[idx[keep]] = acc[keep]
             else:
                 L[idx] = acc
-        for ti in range(t0 + nd, t1):
+        for ti in plan["trsm_tiles"][s0:s1]:
             tl = tiles[ti]
             pn = panels[tl["panel"]]
             nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
