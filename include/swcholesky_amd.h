@@ -121,11 +121,13 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[7] = {panels, tiles, tasks, levels,
- * solve tasks, xsize, TRSM tiles}; swc_plan_export copies array `which`
- * (0 panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->TRSM-list
- * offsets, 5 panels by level, 6 level->panel offsets, 7 solve tasks,
- * 8 panel->solve-task offsets, 9 TRSM tile list) as the packed structs of
+ * device, for tools and tests.  sizes[10] = {panels, tiles, tasks, levels,
+ * solve gathers, xsize, TRSM tiles, solve row blocks, contribution-vector size,
+ * most row blocks in a level}; swc_plan_export copies array `which` (0 panels,
+ * 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->TRSM-list offsets, 5 panels
+ * by level, 6 level->panel offsets, 7 solve gathers, 8 panel->gather offsets,
+ * 9 TRSM tile list, 10 solve row blocks, 11 panel(by level)->row-block offsets,
+ * 12 level->row-block offsets) as the packed structs of
  * swcholesky_amd/csrc/plan.h. */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,

@@ -172,18 +172,19 @@ class Factor:
                     factor_flops=fl[1])
 
 
-PANEL_DT = np.dtype([("vbase", np.int64), ("ld", np.int32), ("ncol", np.int32),
-                     ("nrow", np.int32), ("c0", np.int32), ("rows_off", np.int32),
-                     ("sn", np.int32)])
+PANEL_DT = np.dtype([("vbase", np.int64), ("voff", np.int64), ("ld", np.int32),
+                     ("ncol", np.int32), ("nrow", np.int32), ("c0", np.int32),
+                     ("rows_off", np.int32), ("sn", np.int32), ("sc0", np.int32),
+                     ("sw", np.int32)])
 TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("task_begin", np.int32), ("task_end", np.int32), ("kind", np.int32),
                     ("pad0", np.int32), ("pad1", np.int32)])
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
                     ("ca", np.int32), ("cb", np.int32), ("pad", np.int32)])
-STASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
-                     ("rows_off", np.int32), ("ca", np.int32), ("cb", np.int32),
-                     ("src_c0", np.int32)])
+STASK_DT = np.dtype([("voff", np.int64), ("rows_off", np.int32), ("ca", np.int32),
+                     ("cb", np.int32), ("wd", np.int32)])
+SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("pad", np.int32)])
 
 
 def build_plan(bcsc: BCSC):
@@ -196,22 +197,26 @@ def build_plan(bcsc: BCSC):
     check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
                              ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
     try:
-        sz = np.zeros(7, dtype=np.int64)
+        sz = np.zeros(10, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm = (int(v) for v in sz)
+        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm, nsblk, vsize, maxlb = (
+            int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
                 (6, np.int32, nlevel + 1), (7, STASK_DT, nstask), (8, np.int32, npanel + 1),
-                (9, np.int32, ntrsm)]
+                (9, np.int32, ntrsm), (10, SBLK_DT, nsblk), (11, np.int32, npanel + 1),
+                (12, np.int32, nlevel + 1)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
-                 "lvl_panels", "stasks", "stask_ptr", "trsm_tiles"]
+                 "lvl_panels", "stasks", "stask_ptr", "trsm_tiles", "sblocks", "sblk_ptr",
+                 "lvl_sblk"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)
             check(lib.swc_plan_export(h, which, ptr(a)))
             out[name] = a
-        out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1])
+        out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1],
+                   vsize=vsize, max_lvl_sblk=maxlb)
         return out
     finally:
         lib.swc_plan_free(h)

@@ -246,81 +246,146 @@ k_tile_trsm(double* __restrict__ L, const Panel* __restrict__ panels,
   }
 }
 
-// Forward solve, one panel per workgroup: gather the panel's source
-// contributions, then solve with the panel's diagonal block
-// (Triangular_BCSC.h:14-49 restated as a gather over descendants).
+// ---- triangular solves --------------------------------------------------
+// Forward L y = b (blockedLsolve, Triangular_BCSC.h:14-49, restated level by
+// level): per panel, gather the descendants' contributions V_d = L21(d) x_d,
+// solve the panel's diagonal block, then (row-parallel) push L21 x into the
+// later columns of the same supernode (in place in b) and into V_s.
+// Backward L^T x = y: per panel, row-block partial sums of L21^T x, then a
+// fixed-order reduction and the transposed diagonal solve.
+
+// Loads the panel's nc x nc diagonal block into LDS (column j at dl[j*kLdsLd]).
+__device__ inline void load_diag(const double* __restrict__ L, const Panel& pn, double* dl) {
+  for (int e = threadIdx.x; e < pn.ncol * pn.ncol; e += kThreads) {
+    const int i = e % pn.ncol, j = e / pn.ncol;
+    dl[j * kLdsLd + i] = L[pn.vbase + (int64_t)j * pn.ld + i];
+  }
+}
+
 __global__ void __launch_bounds__(kThreads)
-k_fwd_panel(const double* __restrict__ L, const int* __restrict__ Ls,
-            const Panel* __restrict__ panels, const int* __restrict__ order, int p0,
-            const SolveTask* __restrict__ st, const int* __restrict__ stp,
-            double* __restrict__ X, int nrhs, int64_t ldx) {
-  __shared__ double t[swc::PW];
+k_fwd_diag(const double* __restrict__ L, const int* __restrict__ Ls,
+           const Panel* __restrict__ panels, const int* __restrict__ order, int p0,
+           const SolveTask* __restrict__ st, const int* __restrict__ stp,
+           const double* __restrict__ V, int64_t vld, double* __restrict__ X, int nrhs,
+           int64_t ldx) {
+  __shared__ double dl[swc::PW * kLdsLd];
+  __shared__ double part[kThreads / 64][swc::PW];
   const int q = order[p0 + blockIdx.x];
   const Panel pn = panels[q];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nc = pn.ncol;
+  load_diag(L, pn, dl);
   for (int r = 0; r < nrhs; ++r) {
     double* x = X + (int64_t)r * ldx;
-    for (int j = tid; j < nc; j += kThreads) t[j] = x[pn.c0 + j];
+    const double* v = V + (int64_t)r * vld;
+    part[wave][lane] = 0.0;
     __syncthreads();
-    for (int s = stp[q]; s < stp[q + 1]; ++s) {
+    // wave w gathers sources w, w+4, ...; rows of one source are distinct
+    for (int s = stp[q] + wave; s < stp[q + 1]; s += kThreads / 64) {
       const SolveTask tk = st[s];
-      const double* src = L + tk.src_base;
-      // one thread per source row inside the panel's columns
-      for (int i = tk.ca + tid; i < tk.cb; i += kThreads) {
-        double acc = 0.0;
-        for (int kk = 0; kk < tk.k; ++kk) acc = fma(src[(int64_t)kk * tk.ld + i], x[tk.src_c0 + kk], acc);
-        t[Ls[tk.rows_off + i] - pn.c0] -= acc;
+      for (int i = tk.ca + lane; i < tk.cb; i += 64)
+        part[wave][Ls[tk.rows_off + i] - pn.c0] += v[tk.voff + i - tk.wd];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double ti = 0.0;
+      if (lane < nc) ti = x[pn.c0 + lane] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+      for (int j = 0; j < nc; ++j) {
+        const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
+        if (lane == j) ti = xj;
+        else if (lane > j) ti -= dl[j * kLdsLd + lane] * xj;
       }
-      __syncthreads();
+      if (lane < nc) x[pn.c0 + lane] = ti;
     }
-    const double* d = L + pn.vbase;
-    for (int j = 0; j < nc; ++j) {
-      const double v = t[j] / d[(int64_t)j * pn.ld + j];
-      __syncthreads();
-      if (tid == 0) t[j] = v;
-      for (int i = j + 1 + tid; i < nc; i += kThreads) t[i] -= d[(int64_t)j * pn.ld + i] * v;
-      __syncthreads();
-    }
-    for (int j = tid; j < nc; j += kThreads) x[pn.c0 + j] = t[j];
     __syncthreads();
   }
 }
 
-// Backward solve L^T x = y, one panel per workgroup, levels in reverse order.
 __global__ void __launch_bounds__(kThreads)
-k_bwd_panel(const double* __restrict__ L, const int* __restrict__ Ls,
-            const Panel* __restrict__ panels, const int* __restrict__ order, int p0,
-            double* __restrict__ X, int nrhs, int64_t ldx) {
-  __shared__ double part[kThreads / 64][swc::PW];
-  __shared__ double t[swc::PW];
-  const int q = order[p0 + blockIdx.x];
-  const Panel pn = panels[q];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+k_fwd_update(const double* __restrict__ L, const Panel* __restrict__ panels,
+             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
+             int64_t vld, double* __restrict__ X, int nrhs, int64_t ldx) {
+  __shared__ double xs[swc::PW];
+  __shared__ double part[kThreads / 64][swc::TM];
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
+  const Panel pn = panels[bk.panel];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nc = pn.ncol;
-  const double* blk = L + pn.vbase;
+  const double* blk = L + pn.vbase + bk.r0;
   for (int r = 0; r < nrhs; ++r) {
     double* x = X + (int64_t)r * ldx;
-    // t_j = y_j - sum_{i >= nc} L[i,j] x[row_i]; wave wv takes columns j = wv, wv+4, ...
-    for (int j = wv; j < nc; j += kThreads / 64) {
-      double acc = 0.0;
-      for (int i = nc + lane; i < pn.nrow; i += 64)
-        acc = fma(blk[(int64_t)j * pn.ld + i], x[Ls[pn.rows_off + i]], acc);
+    double* v = V + (int64_t)r * vld;
+    if (tid < nc) xs[tid] = x[pn.c0 + tid];
+    __syncthreads();
+    // wave w sums columns j = w, w+4, ... for row `lane`
+    double acc = 0.0;
+    if (lane < bk.nr)
+      for (int j = wave; j < nc; j += kThreads / 64) acc = fma(blk[(int64_t)j * pn.ld + lane], xs[j], acc);
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && lane < bk.nr) {
+      const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      const int p = (pn.c0 - pn.sc0) + bk.r0 + lane;  // position in the supernode's rows
+      if (p < pn.sw) x[pn.sc0 + p] -= y;               // a later column of the supernode
+      else v[pn.voff + (p - pn.sw)] += y;              // a row of an ancestor
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
+              const Panel* __restrict__ panels, const swc::SolveBlock* __restrict__ blocks,
+              int b0, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
+              int nrhs, int64_t ldx) {
+  __shared__ double xs[swc::TM];
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
+  const Panel pn = panels[bk.panel];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nc = pn.ncol;
+  const double* blk = L + pn.vbase + bk.r0;
+  for (int r = 0; r < nrhs; ++r) {
+    const double* x = X + (int64_t)r * ldx;
+    double* out = Pb + (int64_t)r * pld + (int64_t)blockIdx.x * swc::PW;
+    if (tid < bk.nr) xs[tid] = x[Ls[pn.rows_off + bk.r0 + tid]];
+    __syncthreads();
+    for (int j = wave; j < nc; j += kThreads / 64) {
+      double acc = lane < bk.nr ? blk[(int64_t)j * pn.ld + lane] * xs[lane] : 0.0;
       for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-      if (lane == 0) part[0][j] = acc;
+      if (lane == 0) out[j] = acc;
     }
     __syncthreads();
-    for (int j = tid; j < nc; j += kThreads) t[j] = x[pn.c0 + j] - part[0][j];
-    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ panels,
+           const int* __restrict__ order, int p0, const int* __restrict__ sblk_ptr, int b0,
+           const double* __restrict__ Pb, int64_t pld, double* __restrict__ X, int nrhs,
+           int64_t ldx) {
+  __shared__ double dl[swc::PW * kLdsLd];
+  const int qi = p0 + blockIdx.x;
+  const Panel pn = panels[order[qi]];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nc = pn.ncol;
+  load_diag(L, pn, dl);
+  __syncthreads();
+  if (wave != 0) return;
+  for (int r = 0; r < nrhs; ++r) {
+    double* x = X + (int64_t)r * ldx;
+    const double* pb = Pb + (int64_t)r * pld;
+    double ti = 0.0;
+    if (lane < nc) {
+      double sum = 0.0;
+      for (int b = sblk_ptr[qi]; b < sblk_ptr[qi + 1]; ++b) sum += pb[(int64_t)(b - b0) * swc::PW + lane];
+      ti = x[pn.c0 + lane] - sum;
+    }
     for (int j = nc - 1; j >= 0; --j) {
-      if (tid == 0) t[j] = t[j] / blk[(int64_t)j * pn.ld + j];
-      __syncthreads();
-      const double v = t[j];
-      for (int i = tid; i < j; i += kThreads) t[i] -= blk[(int64_t)i * pn.ld + j] * v;
-      __syncthreads();
+      const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
+      if (lane == j) ti = xj;
+      else if (lane < j) ti -= dl[lane * kLdsLd + j] * xj;
     }
-    for (int j = tid; j < nc; j += kThreads) x[pn.c0 + j] = t[j];
-    __syncthreads();
+    if (lane < nc) x[pn.c0 + lane] = ti;
   }
 }
 
@@ -361,6 +426,10 @@ struct swc_handle {
   Task* dTasks = nullptr;
   SolveTask* dSt = nullptr;
   int* dStp = nullptr;
+  swc::SolveBlock* dSblk = nullptr;
+  int* dSblkPtr = nullptr;
+  double* dV = nullptr;   // contribution vectors, vsize * work_cols
+  double* dPb = nullptr;  // backward partial sums, max_lvl_sblk * PW * work_cols
   int* dOrder = nullptr;
   int* dTrsm = nullptr;
   int64_t* dAmap = nullptr;
@@ -381,10 +450,10 @@ struct swc_handle {
   ~swc_handle() { release(); }
   void release() {
     hipFree(dL); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
-    hipFree(dSt); hipFree(dStp); hipFree(dOrder); hipFree(dTrsm); hipFree(dAmap); hipFree(dPerm);
+    hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
     dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dSt = nullptr; dStp = nullptr; dOrder = nullptr; dTrsm = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
@@ -444,8 +513,14 @@ int build_amap(const swc_handle* h, const int* a2p, const int* a2i, const int64_
 int ensure_work(swc_handle* h, int cols) {
   if (h->work_cols >= cols) return SWC_OK;
   hipFree(h->dWork);
-  h->dWork = nullptr;
+  hipFree(h->dV);
+  hipFree(h->dPb);
+  h->dWork = h->dV = h->dPb = nullptr;
+  h->work_cols = 0;
+  const Plan& P = h->plan;
   SWC_HIP(hipMalloc((void**)&h->dWork, sizeof(double) * (size_t)h->n * cols));
+  SWC_HIP(hipMalloc((void**)&h->dV, sizeof(double) * (size_t)std::max<int64_t>(P.vsize, 1) * cols));
+  SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_sblk, 1) * swc::PW * cols));
   h->work_cols = cols;
   return SWC_OK;
 }
@@ -513,24 +588,40 @@ int factor_device(swc_handle* h, const double* dvals) {
   return SWC_OK;
 }
 
-// In-place solve of L L^T X = B on device columns X (factor ordering).
+// In-place solve of L L^T X = B on device columns X (factor ordering); needs
+// ensure_work(h, nrhs).
 int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
+  const int64_t vld = std::max<int64_t>(P.vsize, 1);
+  const int64_t pld = (int64_t)std::max(P.max_lvl_sblk, 1) * swc::PW;
   SWC_HIP(hipEventRecord(h->ev[2], st));
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
   for (int l = 0; l < P.nlevel; ++l) {
     const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
+    const int b0 = P.lvl_sblk[l], nb = P.lvl_sblk[l + 1] - b0;
     if (np > 0)
-      hipLaunchKernelGGL(k_fwd_panel, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_fwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
-                         (const SolveTask*)h->dSt, (const int*)h->dStp, dX, nrhs, ldx);
+                         (const SolveTask*)h->dSt, (const int*)h->dStp, (const double*)h->dV, vld,
+                         dX, nrhs, ldx);
+    if (nb > 0)
+      hipLaunchKernelGGL(k_fwd_update, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
+                         (const Panel*)h->dPanels, (const swc::SolveBlock*)h->dSblk, b0, h->dV,
+                         vld, dX, nrhs, ldx);
   }
   for (int l = P.nlevel - 1; l >= 0; --l) {
     const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
+    const int b0 = P.lvl_sblk[l], nb = P.lvl_sblk[l + 1] - b0;
+    if (nb > 0)
+      hipLaunchKernelGGL(k_bwd_partial, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
+                         (const int*)h->dLs, (const Panel*)h->dPanels,
+                         (const swc::SolveBlock*)h->dSblk, b0, h->dPb, pld, (const double*)dX,
+                         nrhs, ldx);
     if (np > 0)
-      hipLaunchKernelGGL(k_bwd_panel, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
-                         dX, nrhs, ldx);
+      hipLaunchKernelGGL(k_bwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
+                         (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
+                         (const int*)h->dSblkPtr, b0, (const double*)h->dPb, pld, dX, nrhs, ldx);
   }
   SWC_HIP(hipEventRecord(h->ev[3], st));
   SWC_HIP(hipGetLastError());
@@ -612,6 +703,8 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dTasks, P.tasks.data(), P.tasks.size(), st);
   if (!rc) rc = upload(&h->dSt, P.stasks.data(), P.stasks.size(), st);
   if (!rc) rc = upload(&h->dStp, P.stask_ptr.data(), P.stask_ptr.size(), st);
+  if (!rc) rc = upload(&h->dSblk, P.sblocks.data(), P.sblocks.size(), st);
+  if (!rc) rc = upload(&h->dSblkPtr, P.sblk_ptr.data(), P.sblk_ptr.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
   if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
   if (!rc) rc = upload(&h->dAmap, amap.data(), amap.size(), st);

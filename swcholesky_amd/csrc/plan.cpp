@@ -25,9 +25,11 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   P.n = n;
   P.nsuper = nsuper;
   std::vector<int> first_panel(nsuper + 1, 0);
+  std::vector<int64_t> voff(nsuper + 1, 0);
   for (int s = 0; s < nsuper; ++s) {
     const int W = super[s + 1] - super[s];
     const int64_t R = pi[s + 1] - pi[s];
+    voff[s + 1] = voff[s] + (R - W);
     if (W <= 0 || R < W) throw std::invalid_argument("inconsistent supernode structure");
     if (R > 0x7fffffff) throw std::invalid_argument("supernode too tall");
     first_panel[s] = (int)P.panels.size();
@@ -40,12 +42,16 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       p.c0 = super[s] + q * PW;
       p.rows_off = (int)(pi[s] + q * PW);
       p.sn = s;
+      p.voff = voff[s];
+      p.sc0 = super[s];
+      p.sw = W;
       P.panels.push_back(p);
     }
     P.xsize = std::max<int64_t>(P.xsize, px[s] + (int64_t)W * R);
   }
   first_panel[nsuper] = (int)P.panels.size();
   P.npanel = (int)P.panels.size();
+  P.vsize = voff[nsuper];
 
   // ---- descendant sources of each panel ----
   struct Src { int d; int ca, cb; };  // d >= 0: supernode, d < 0: panel -(d+1) of same sn
@@ -74,7 +80,6 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       t.ld = (int)(pi[d + 1] - pi[d]);
       t.k = super[d + 1] - super[d];
       t.rows_off = (int)pi[d];
-      st.src_c0 = super[d];
     } else {
       const Panel& sp = P.panels[-(sr.d + 1)];
       const int s = pn.sn;
@@ -83,37 +88,26 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       t.ld = pn.ld;
       t.k = sp.ncol;
       t.rows_off = (int)pi[s];
-      st.src_c0 = sp.c0;
     }
     t.ca = sr.ca;
     t.cb = sr.cb;
     t.pad = 0;
-    st.src_base = t.src_base;
-    st.ld = t.ld;
-    st.k = t.k;
-    st.rows_off = t.rows_off;
-    st.ca = sr.ca;
-    st.cb = sr.cb;
+    (void)st;
   };
 
-  // ---- forward-solve sources: descendants, then earlier panels of the supernode ----
+  // ---- forward-solve gathers: descendants touching each panel ----
   P.stask_ptr.assign(P.npanel + 1, 0);
-  for (int s = 0; s < nsuper; ++s) {
-    const int fp = first_panel[s], lp = first_panel[s + 1];
-    for (int q = fp; q < lp; ++q) {
-      Task t;
+  for (int q = 0; q < P.npanel; ++q) {
+    for (const Src& sr : srcs[q]) {
       SolveTask st;
-      for (const Src& sr : srcs[q]) {
-        describe(q, sr, t, st);
-        P.stasks.push_back(st);
-      }
-      for (int q2 = fp; q2 < q; ++q2) {
-        const int off = (q - fp) * PW;
-        describe(q, {-(q2 + 1), off, off + P.panels[q].ncol}, t, st);
-        P.stasks.push_back(st);
-      }
-      P.stask_ptr[q + 1] = (int)P.stasks.size();
+      st.voff = voff[sr.d];
+      st.rows_off = (int)pi[sr.d];
+      st.ca = sr.ca;
+      st.cb = sr.cb;
+      st.wd = super[sr.d + 1] - super[sr.d];
+      P.stasks.push_back(st);
     }
+    P.stask_ptr[q + 1] = (int)P.stasks.size();
   }
 
   // ---- levels: tiles of each level form one update launch, followed by one
@@ -217,6 +211,21 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
   }
 
+  // ---- solve row blocks below each panel's diagonal block, panels by level ----
+  P.sblk_ptr.assign(P.npanel + 1, 0);
+  P.lvl_sblk.assign(P.nlevel + 1, 0);
+  for (int l = 0; l < P.nlevel; ++l) {
+    for (int qi = P.lvl_panels[l]; qi < P.lvl_panels[l + 1]; ++qi) {
+      const Panel& pn = P.panels[P.panel_order[qi]];
+      P.sblk_ptr[qi] = (int)P.sblocks.size();
+      for (int r0 = pn.ncol; r0 < pn.nrow; r0 += TM)
+        P.sblocks.push_back({P.panel_order[qi], r0, std::min(TM, pn.nrow - r0), 0});
+    }
+    P.lvl_sblk[l + 1] = (int)P.sblocks.size();
+    P.max_lvl_sblk = std::max(P.max_lvl_sblk, P.lvl_sblk[l + 1] - P.lvl_sblk[l]);
+  }
+  P.sblk_ptr[P.npanel] = (int)P.sblocks.size();
+
   // ---- flatten: tiles by level (stable), tasks, TRSM lists ----
   std::vector<int> order(ptiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
@@ -304,6 +313,9 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[4] = (int64_t)P.stasks.size();
   out[5] = P.xsize;
   out[6] = (int64_t)P.trsm_tiles.size();
+  out[7] = (int64_t)P.sblocks.size();
+  out[8] = P.vsize;
+  out[9] = P.max_lvl_sblk;
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
@@ -329,6 +341,9 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 7: return cp(P.stasks);
     case 8: return cp(P.stask_ptr);
     case 9: return cp(P.trsm_tiles);
+    case 10: return cp(P.sblocks);
+    case 11: return cp(P.sblk_ptr);
+    case 12: return cp(P.lvl_sblk);
     default: return swc_set_error(SWC_EINVAL, "swc_plan_export: unknown array id");
   }
 }

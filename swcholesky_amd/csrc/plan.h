@@ -22,12 +22,15 @@ constexpr int TM = 64;  // tile height (rows)
 
 struct Panel {
   int64_t vbase;  // value offset of (panel row 0, panel column 0)
+  int64_t voff;   // offset of the supernode's contribution vector (solves)
   int ld;         // leading dimension (row count of the supernode)
   int ncol;       // columns in the panel (<= PW)
   int nrow;       // rows from the panel's first column down
   int c0;         // first global column
   int rows_off;   // offset into Ls of the panel's row 0
   int sn;         // owning supernode
+  int sc0;        // first column of the supernode
+  int sw;         // width of the supernode
 };
 
 struct Tile {
@@ -52,14 +55,22 @@ struct Task {
   int pad;
 };
 
-// Forward/backward solve contributions into one panel: source rows [ca,cb)
-// (rows inside the panel's columns) times the source's solved columns.
+// Forward-solve gather into one panel from a descendant supernode d: rows
+// [ca,cb) of d's row list lie in the panel's columns; their values are taken
+// from d's contribution vector V_d = L21(d) x_d at V[voff + i - wd].
 struct SolveTask {
-  int64_t src_base;
-  int ld, k;
+  int64_t voff;
   int rows_off;
   int ca, cb;
-  int src_c0;  // first global column of the source's k columns
+  int wd;
+};
+
+// Rows [r0, r0+nr) of a panel below its diagonal block (solve mat-vecs).
+struct SolveBlock {
+  int panel;
+  int r0;
+  int nr;
+  int pad;
 };
 
 struct Plan {
@@ -74,8 +85,13 @@ struct Plan {
   std::vector<int> panel_level;     // per panel
   std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order
   std::vector<int> panel_order;     // panels grouped by level
-  std::vector<SolveTask> stasks;    // per panel, sources for the forward solve
+  std::vector<SolveTask> stasks;    // per panel, descendant gathers of the forward solve
   std::vector<int> stask_ptr;       // npanel+1
+  std::vector<SolveBlock> sblocks;  // row blocks below each panel's diagonal, by level
+  std::vector<int> sblk_ptr;        // npanel+1 into sblocks (panels in panel_order)
+  std::vector<int> lvl_sblk;        // nlevel+1 into sblocks
+  int max_lvl_sblk = 0;             // most row blocks in one level
+  int64_t vsize = 0;                // sum over supernodes of (rows - width)
   double update_flops = 0, factor_flops = 0;
   // algorithmic flops of each kernel class: lower-triangle update products and
   // the diagonal-tile POTRF + in-tile TRSM (tile kernel), off-diagonal TRSM

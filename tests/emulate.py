@@ -82,32 +82,71 @@ def factor(plan, Ls, xsize, amap, vals):
 
 
 def solve(plan, Ls, L, b):
-    """Forward then backward solve in the factor ordering (b is n-vector)."""
+    """Forward then backward solve in the factor ordering (b is an n-vector),
+    as k_fwd_diag / k_fwd_update / k_bwd_partial / k_bwd_diag run it."""
     x = np.array(b, dtype=np.float64)
+    n = len(x)
     panels, st, stp, order = plan["panels"], plan["stasks"], plan["stask_ptr"], plan["panel_order"]
-    lp = plan["lvl_panels"]
+    lp, blocks, lb, sbp = plan["lvl_panels"], plan["sblocks"], plan["lvl_sblk"], plan["sblk_ptr"]
+    vsize = plan["vsize"]
+    V = np.zeros(max(vsize, 1))
+    xsize = len(L)
+
+    def diag(pn):
+        nc, ld = int(pn["ncol"]), int(pn["ld"])
+        idx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
+        _chk(int(idx.min()), int(idx.max()) + 1, xsize, "diag")
+        return np.tril(L[idx])
+
     for lvl in range(plan["nlevel"]):
         for qi in range(lp[lvl], lp[lvl + 1]):
             q = order[qi]
             pn = panels[q]
-            nc, ld, c0 = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"])
+            nc, c0 = int(pn["ncol"]), int(pn["c0"])
             t = x[c0:c0 + nc].copy()
-            for s in range(stp[q], stp[q + 1]):
-                tk = st[s]
-                ca, cb, k, sld, sb = (int(tk[f]) for f in ("ca", "cb", "k", "ld", "src_base"))
-                sc0, roff = int(tk["src_c0"]), int(tk["rows_off"])
-                A = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
-                pos = Ls[roff + ca:roff + cb] - c0
+            for s_ in range(stp[q], stp[q + 1]):
+                tk = st[s_]
+                ca, cb, ro, wd, vo = (int(tk[f]) for f in ("ca", "cb", "rows_off", "wd", "voff"))
+                assert ca >= wd
+                _chk(vo + ca - wd, vo + cb - wd, vsize, "V gather")
+                pos = Ls[ro + ca:ro + cb] - c0
                 assert np.all((pos >= 0) & (pos < nc))
-                t[pos] -= A @ x[sc0:sc0 + k]
-            d = L[int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]]
-            x[c0:c0 + nc] = np.linalg.solve(np.tril(d), t)
+                t[pos] -= V[vo + np.arange(ca, cb) - wd]
+            x[c0:c0 + nc] = np.linalg.solve(diag(pn), t)
+        for bk in blocks[lb[lvl]:lb[lvl + 1]]:
+            pn = panels[bk["panel"]]
+            nc, ld, c0 = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"])
+            r0, nr = int(bk["r0"]), int(bk["nr"])
+            assert 1 <= nr <= 64 and r0 >= nc and r0 + nr <= int(pn["nrow"])
+            blk = L[int(pn["vbase"]) + r0 + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]]
+            y = blk @ x[c0:c0 + nc]
+            p = (c0 - int(pn["sc0"])) + r0 + np.arange(nr)
+            intra = p < int(pn["sw"])
+            tgt = int(pn["sc0"]) + p[intra]
+            _chk(int(tgt.min(initial=0)), int(tgt.max(initial=0)) + 1, n, "intra")
+            x[tgt] -= y[intra]
+            vi = int(pn["voff"]) + p[~intra] - int(pn["sw"])
+            if vi.size:
+                _chk(int(vi.min()), int(vi.max()) + 1, vsize, "V push")
+            V[vi] += y[~intra]
     for lvl in range(plan["nlevel"] - 1, -1, -1):
+        b0 = lb[lvl]
+        part = {}
+        for bi in range(lb[lvl], lb[lvl + 1]):
+            bk = blocks[bi]
+            pn = panels[bk["panel"]]
+            nc, ld = int(pn["ncol"]), int(pn["ld"])
+            r0, nr = int(bk["r0"]), int(bk["nr"])
+            assert bi - b0 < plan["max_lvl_sblk"]
+            blk = L[int(pn["vbase"]) + r0 + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]]
+            rows = Ls[int(pn["rows_off"]) + r0:int(pn["rows_off"]) + r0 + nr]
+            part[bi] = blk.T @ x[rows]
         for qi in range(lp[lvl], lp[lvl + 1]):
             pn = panels[order[qi]]
-            nc, ld, c0, nrow = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"]), int(pn["nrow"])
-            blk = L[int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nrow)[:, None]]
-            rows = Ls[int(pn["rows_off"]):int(pn["rows_off"]) + nrow]
-            t = x[c0:c0 + nc] - blk[nc:, :].T @ x[rows[nc:]]
-            x[c0:c0 + nc] = np.linalg.solve(np.tril(blk[:nc, :]).T, t)
+            nc, c0 = int(pn["ncol"]), int(pn["c0"])
+            t = x[c0:c0 + nc].copy()
+            for bi in range(sbp[qi], sbp[qi + 1]):
+                assert lb[lvl] <= bi < lb[lvl + 1]
+                t -= part[bi]
+            x[c0:c0 + nc] = np.linalg.solve(diag(pn).T, t)
     return x

@@ -20,10 +20,13 @@ def _amap(b):
 
 
 @pytest.mark.parametrize("make", [lambda: P.laplacian2d_const(10), lambda: P.laplacian2d_var(37, 3),
-                                  lambda: P.laplacian3d_var(12), lambda: P.vector3d_var(6)])
+                                  lambda: P.laplacian3d_var(12), lambda: P.vector3d_var(6),
+                                  lambda: P.laplacian2d_const(100), lambda: P.vector3d_var(9)])
 def test_emulated_kernels_match_oracle(make):
     prob = make()
     b, a2x, Lo, _ = setup(prob)
+    if b.n >= 2000:  # these cases have supernodes wider than one panel
+        assert b.maxSupWid > 64
     plan = api.build_plan(b)
     Le, info = E.factor(plan, b.s, b.xsize, _amap(b), a2x)
     assert info == 0
