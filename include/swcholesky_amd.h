@@ -99,6 +99,14 @@ int swc_upload_symbolic(swc_handle* h, const swc_symbolic* sym);
  * NULL clears it (solves then work in the factor's ordering). */
 int swc_set_perm(swc_handle* h, const int* perm);
 
+/* Precision of the next swc_factor calls.  0 (default): FP64 throughout.
+ * 1: descendant SYRK/GEMM updates on FP32 MFMA with FP32 accumulation, each
+ *    update subtracted into the FP64 target; POTRF, TRSM and the updates inside
+ *    a supernode's own diagonal block stay FP64.
+ * 2: as 1, and the updates inside wide supernodes also use FP32 MFMA; only the
+ *    64x64 diagonal-tile POTRF and the TRSMs stay FP64.
+ * L is stored in FP64 in every mode; swc_refine recovers FP64 accuracy. */
+int swc_set_precision(swc_handle* h, int mode);
 /* Numeric factorization.  values: host or device pointer (values_on_device).
  * *info = 0, or k+1 for the first column k whose pivot is not positive
  * (LAPACK convention); returns SWC_ENOTPOS in that case. */
@@ -109,6 +117,19 @@ int swc_get_factor(swc_handle* h, double* lValues, int on_device);
  * permutation, forward L y = P b (blockedLsolve, triangularSolve/Triangular_BCSC.h:14-49),
  * backward L^T z = y (no reference counterpart), inverse permutation. */
 int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device);
+/* Solves A X = B to backward error eta = ||b - A x||_inf / (||A||_inf ||x||_inf +
+ * ||b||_inf) <= tol per column, by FP64 refinement on the current factor: the
+ * residual is an FP64 SpMV with the values given to swc_factor, corrections use
+ * the factor.  method 0: classical refinement; 1: conjugate gradients
+ * preconditioned by L L^T; -1: 1 after a mixed-precision factor, else 0.
+ * B and X (n x nrhs, leading dimension ld) may alias; eta[nrhs] receives the
+ * final backward errors, *iters the number of corrections.  No reference
+ * counterpart (the reference has no solve after factorization). */
+int swc_refine(swc_handle* h, const double* B, double* X, int nrhs, int64_t ld, int on_device,
+               double tol, int maxiter, int method, double* eta, int* iters);
+/* eta[c] of a given X for A X = B (FP64, on the GPU). */
+int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int64_t ld,
+                 int on_device, double* eta);
 /* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
 int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
 /* With profiling on, every kernel launch of swc_factor is bracketed by HIP events

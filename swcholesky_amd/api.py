@@ -153,6 +153,42 @@ class Factor:
         check(_lib.lib().swc_solve(self._h, p, nrhs, self.n, dev))
         return B
 
+    def set_precision(self, mode):
+        """0: FP64; 1: FP32-MFMA descendant updates; 2: FP32 for every update."""
+        check(_lib.lib().swc_set_precision(self._h, int(mode)))
+
+    def refine(self, b, tol=1e-12, maxiter=50, method=-1):
+        """x with backward error <= tol by FP64 refinement; returns (x, eta, iters).
+
+        b: numpy vector / n x nrhs matrix, or a device tensor laid out n-major per
+        column (torch tensor of shape (nrhs, n) or (n,)), refined in place."""
+        lib = _lib.lib()
+        if isinstance(b, np.ndarray):
+            vec = b.ndim == 1
+            B = np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(self.n, -1))
+            X = np.zeros_like(B, order="F")
+            nrhs = B.shape[1]
+            eta = np.zeros(nrhs)
+            it = ctypes.c_int(0)
+            check(lib.swc_refine(self._h, ptr(B), ptr(X), nrhs, self.n, 0, float(tol),
+                                 int(maxiter), int(method), ptr(eta), ctypes.byref(it)))
+            return (X[:, 0] if vec else X), eta, it.value
+        p, dev = _dptr(b)
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        eta = np.zeros(nrhs)
+        it = ctypes.c_int(0)
+        check(lib.swc_refine(self._h, p, p, nrhs, self.n, dev, float(tol), int(maxiter),
+                             int(method), ptr(eta), ctypes.byref(it)))
+        return b, eta, it.value
+
+    def backward_error(self, b, x):
+        """eta per column of x for A x = b, computed on the GPU in FP64."""
+        B = np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(self.n, -1))
+        X = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(self.n, -1))
+        eta = np.zeros(B.shape[1])
+        check(_lib.lib().swc_residual(self._h, ptr(B), ptr(X), B.shape[1], self.n, 0, ptr(eta)))
+        return eta
+
     def set_profiling(self, on=True):
         check(_lib.lib().swc_set_profiling(self._h, int(on)))
 
@@ -181,7 +217,7 @@ TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("pad0", np.int32), ("pad1", np.int32)])
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
-                    ("ca", np.int32), ("cb", np.int32), ("pad", np.int32)])
+                    ("ca", np.int32), ("cb", np.int32), ("kind", np.int32)])
 STASK_DT = np.dtype([("voff", np.int64), ("rows_off", np.int32), ("ca", np.int32),
                      ("cb", np.int32), ("wd", np.int32)])
 SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("pad", np.int32)])

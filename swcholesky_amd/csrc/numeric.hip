@@ -112,6 +112,43 @@ __device__ inline void task_mfma(const double* __restrict__ src, int ld, int k, 
   }
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The same product with operands rounded to FP32 and FP32 accumulation
+// (v_mfma_f32_16x16x4f32, D[4*(l/16)+r][l%16] in component r); the caller
+// subtracts the FP32 result from its FP64 accumulator.
+__device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int k, int ra, int M,
+                                     int ca, int N, int wave, int lane, f4 (&c)[2][2]) {
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+  if (wr >= M || wc >= N) return;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = wr + lr, r1 = wr + 16 + lr;
+  const int q0 = wc + lr, q1 = wc + 16 + lr;
+  const bool a0ok = r0 < M, a1ok = r1 < M, b0ok = q0 < N, b1ok = q1 < N;
+  const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;
+  const double* pa = src + ra;
+  const double* pb = src + ca;
+  for (int kk = 0; kk < k; kk += 4) {
+    const int kc = kk + lk;
+    const bool kok = kc < k;
+    const int64_t off = (int64_t)kc * ld;
+    const float a0 = (a0ok && kok) ? (float)pa[off + r0] : 0.f;
+    const float a1 = (a1ok && kok) ? (float)pa[off + r1] : 0.f;
+    const float b0 = (b0ok && kok) ? (float)pb[off + q0] : 0.f;
+    const float b1 = (b1ok && kok) ? (float)pb[off + q1] : 0.f;
+    c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, c[0][0], 0, 0, 0);
+    if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, c[0][1], 0, 0, 0);
+    if (blk_a1) {
+      c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, c[1][0], 0, 0, 0);
+      if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[1][1], 0, 0, 0);
+    }
+  }
+}
+
 // Tile kernel: accumulate all source updates of one tile in LDS (in task order),
 // then, for a diagonal tile, factor the panel's diagonal block and solve the
 // tile's remaining rows against it.  info_out receives min(first failing
@@ -119,7 +156,8 @@ __device__ inline void task_mfma(const double* __restrict__ src, int ld, int k, 
 __global__ void __launch_bounds__(kThreads)
 k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
                      const Panel* __restrict__ panels, const Tile* __restrict__ tiles,
-                     const Task* __restrict__ tasks, int tile0, int* __restrict__ info_out) {
+                     const Task* __restrict__ tasks, int tile0, int* __restrict__ info_out,
+                     int fp32_mask) {
   __shared__ double acc[swc::PW * kLdsLd];
   __shared__ int trow[swc::TM];
   __shared__ int pr[swc::TM];
@@ -156,6 +194,23 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
         for (int kk = 0; kk < tk.k; ++kk) s = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], s);
         acc[tj * kLdsLd + ti] -= s;
       }
+    } else if ((fp32_mask >> tk.kind) & 1) {
+      f4 c[2][2];
+      task_mfma_f32(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
+      const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = wr + 16 * bi + 4 * (lane >> 4) + r;
+            const int j = wc + 16 * bj + (lane & 15);
+            if (i < M && j < N) {
+              const int ti = pr[i], tj = pc[j];
+              if (trow[ti] >= pn.c0 + tj) acc[tj * kLdsLd + ti] -= (double)c[bi][bj][r];
+            }
+          }
     } else {
       d4 c[2][2];
       task_mfma(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
@@ -405,6 +460,123 @@ __global__ void k_permute(const double* __restrict__ in, double* __restrict__ ou
 
 }  // namespace
 
+// ---- residual, norms and vector updates (FP64) ---------------------------
+
+constexpr int kRedBlocks = 512;
+
+// Af[e] = vals[vmap[e]]: the full P A P^T values in CSR order, from the caller's values.
+__global__ void k_gather(double* __restrict__ out, const double* __restrict__ vals,
+                         const int64_t* __restrict__ vmap, int64_t nnz) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = vals[vmap[e]];
+}
+
+// R = B - A X for column blockIdx.y (CSR, one thread per row).
+__global__ void k_residual(const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ av, const double* __restrict__ X,
+                           const double* __restrict__ B, double* __restrict__ R, int n,
+                           int64_t ld) {
+  const int64_t c = blockIdx.y;
+  const double* x = X + c * ld;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double t = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) t = fma(av[e], x[ci[e]], t);
+    R[c * ld + i] = B[c * ld + i] - t;
+  }
+}
+
+// y = A x for column blockIdx.y.
+__global__ void k_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
+                       const double* __restrict__ av, const double* __restrict__ X,
+                       double* __restrict__ Y, int n, int64_t ld) {
+  const int64_t c = blockIdx.y;
+  const double* x = X + c * ld;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double t = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) t = fma(av[e], x[ci[e]], t);
+    Y[c * ld + i] = t;
+  }
+}
+
+// ||A||_inf = max_i sum_j |a_ij|, written to out[0] (single block, deterministic).
+__global__ void k_norm_inf(const int* __restrict__ rp, const double* __restrict__ av, int n,
+                           double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) {
+    double t = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) t += fabs(av[e]);
+    m = fmax(m, t);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Per-column partial reductions over a fixed grid (deterministic): op 0 max|x|,
+// op 1 dot(x, y).  part[c * kRedBlocks + block].
+__global__ void k_reduce_partial(const double* __restrict__ X, const double* __restrict__ Y,
+                                 int n, int64_t ld, int op, double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  const int64_t c = blockIdx.y;
+  const double* x = X + c * ld;
+  const double* y = Y ? Y + c * ld : nullptr;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += kRedBlocks * kThreads)
+    acc = op == 0 ? fmax(acc, fabs(x[i])) : fma(x[i], y[i], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      red[threadIdx.x] = op == 0 ? fmax(red[threadIdx.x], red[threadIdx.x + w])
+                                 : red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[c * kRedBlocks + blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int op, double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += kThreads) {
+    const double v = part[(int64_t)c * kRedBlocks + i];
+    acc = op == 0 ? fmax(acc, v) : acc + v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      red[threadIdx.x] = op == 0 ? fmax(red[threadIdx.x], red[threadIdx.x + w])
+                                 : red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// Y[:, c] += alpha[c] * X[:, c]   (sign folded into alpha by the caller)
+__global__ void k_axpy_cols(double* __restrict__ Y, const double* __restrict__ X,
+                            const double* __restrict__ alpha, int n, int64_t ld) {
+  const int64_t c = blockIdx.y;
+  const double a = alpha[c];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    Y[c * ld + i] = fma(a, X[c * ld + i], Y[c * ld + i]);
+}
+
+// P[:, c] = Z[:, c] + beta[c] * P[:, c]
+__global__ void k_xpby_cols(double* __restrict__ P, const double* __restrict__ Z,
+                            const double* __restrict__ beta, int n, int64_t ld) {
+  const int64_t c = blockIdx.y;
+  const double b = beta[c];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    P[c * ld + i] = fma(b, P[c * ld + i], Z[c * ld + i]);
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -438,6 +610,19 @@ struct swc_handle {
   double* dWork = nullptr;  // solve workspace, n * work_cols doubles
   int work_cols = 0;
   int* dInfo = nullptr;
+  // full P A P^T (factor ordering) for residuals: CSR pattern, value map, values
+  int* dRp = nullptr;
+  int* dCi = nullptr;
+  int64_t* dVmap = nullptr;
+  double* dAf = nullptr;
+  int64_t nnz_full = 0;
+  double* dScal = nullptr;   // small device scalars (norms, reductions)
+  double* dPart = nullptr;   // reduction partials, kRedBlocks * work_cols
+  double* dRef = nullptr;    // refinement vectors, 5 * n * work_cols
+  double normA = 0;
+  int precision = 0;  // 0 FP64; 1 FP32 descendant updates; 2 FP32 for every update
+  int fp32_mask = 0;  // bit k: tasks of kind k use FP32 MFMA
+  int factored_precision = 0;
   bool factored = false;
   int last_info = 0;
   // timing (device events on `stream`)
@@ -452,9 +637,13 @@ struct swc_handle {
     hipFree(dL); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
     hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
+    hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart);
+    hipFree(dRef);
     dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
     dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
+    dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr;
+    dRef = nullptr;
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     for (auto& e : pev) hipEventDestroy(e);
@@ -515,9 +704,16 @@ int ensure_work(swc_handle* h, int cols) {
   hipFree(h->dWork);
   hipFree(h->dV);
   hipFree(h->dPb);
-  h->dWork = h->dV = h->dPb = nullptr;
+  hipFree(h->dPart);
+  hipFree(h->dRef);
+  h->dWork = h->dV = h->dPb = h->dPart = h->dRef = nullptr;
   h->work_cols = 0;
   const Plan& P = h->plan;
+  SWC_HIP(hipMalloc((void**)&h->dPart, sizeof(double) * (size_t)kRedBlocks * cols));
+  hipFree(h->dScal);
+  h->dScal = nullptr;
+  SWC_HIP(hipMalloc((void**)&h->dScal, sizeof(double) * (size_t)(64 + 4 * cols)));
+  SWC_HIP(hipMalloc((void**)&h->dRef, sizeof(double) * (size_t)5 * h->n * cols));
   SWC_HIP(hipMalloc((void**)&h->dWork, sizeof(double) * (size_t)h->n * cols));
   SWC_HIP(hipMalloc((void**)&h->dV, sizeof(double) * (size_t)std::max<int64_t>(P.vsize, 1) * cols));
   SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_sblk, 1) * swc::PW * cols));
@@ -538,6 +734,10 @@ int factor_device(swc_handle* h, const double* dvals) {
   hipLaunchKernelGGL(k_fill, dim3(grid_for(P.xsize)), dim3(kThreads), 0, st, h->dL, P.xsize, 0.0);
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(h->nvals)), dim3(kThreads), 0, st, h->dL,
                      (const int64_t*)h->dAmap, dvals, h->nvals);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(h->nnz_full)), dim3(kThreads), 0, st, h->dAf, dvals,
+                     (const int64_t*)h->dVmap, h->nnz_full);
+  hipLaunchKernelGGL(k_norm_inf, dim3(1), dim3(kThreads), 0, st, (const int*)h->dRp,
+                     (const double*)h->dAf, h->n, h->dScal);
   const int big = 0x7fffffff;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, big);
   for (int l = 0; l < P.nlevel; ++l) {
@@ -548,7 +748,7 @@ int factor_device(swc_handle* h, const double* dvals) {
     if (nt > 0)
       hipLaunchKernelGGL(k_tile_update_factor, dim3(nt), dim3(kThreads), 0, st, h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0, h->dInfo);
+                         (const Task*)h->dTasks, t0, h->dInfo, h->fp32_mask);
     if (pe) SWC_HIP(hipEventRecord(pe[1], st));
     if (ns > 0)
       hipLaunchKernelGGL(k_tile_trsm, dim3(ns), dim3(kThreads), 0, st, h->dL,
@@ -560,6 +760,7 @@ int factor_device(swc_handle* h, const double* dvals) {
   SWC_HIP(hipGetLastError());
   int info = 0;
   SWC_HIP(hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st));
+  SWC_HIP(hipMemcpyAsync(&h->normA, h->dScal, sizeof(double), hipMemcpyDeviceToHost, st));
   SWC_HIP(hipStreamSynchronize(st));
   float ms = 0;
   SWC_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
@@ -585,6 +786,7 @@ int factor_device(swc_handle* h, const double* dvals) {
   }
   h->last_info = info == big ? 0 : info;
   h->factored = true;
+  h->factored_precision = h->precision;
   return SWC_OK;
 }
 
@@ -625,6 +827,164 @@ int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   }
   SWC_HIP(hipEventRecord(h->ev[3], st));
   SWC_HIP(hipGetLastError());
+  return SWC_OK;
+}
+
+// ---- vectors in and out of the factor ordering ----------------------------
+
+// dOut (n x nrhs, ld n, factor ordering) = P B; B host or device with ld ldb.
+int to_factor(swc_handle* h, const double* B, int on_device, int64_t ldb, double* dOut, int nrhs) {
+  const int n = h->n;
+  hipStream_t st = h->stream;
+  const double* src = B;
+  int64_t lds = ldb;
+  if (!on_device) {
+    SWC_HIP(hipMemcpy2DAsync(h->dWork, sizeof(double) * n, B, sizeof(double) * ldb,
+                             sizeof(double) * n, nrhs, hipMemcpyHostToDevice, st));
+    src = h->dWork;
+    lds = n;
+  }
+  if (!h->perm.empty())
+    hipLaunchKernelGGL(k_permute, dim3(grid_for((int64_t)n * nrhs)), dim3(kThreads), 0, st, src,
+                       dOut, (const int*)h->dPerm, n, nrhs, lds, (int64_t)n, 0);
+  else
+    SWC_HIP(hipMemcpy2DAsync(dOut, sizeof(double) * n, src, sizeof(double) * lds,
+                             sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, st));
+  return SWC_OK;
+}
+
+// X (host or device, ld ldx) = P^T dIn.
+int from_factor(swc_handle* h, const double* dIn, double* X, int on_device, int64_t ldx, int nrhs) {
+  const int n = h->n;
+  hipStream_t st = h->stream;
+  double* dst = on_device ? X : h->dWork;
+  const int64_t ldd = on_device ? ldx : n;
+  if (!h->perm.empty())
+    hipLaunchKernelGGL(k_permute, dim3(grid_for((int64_t)n * nrhs)), dim3(kThreads), 0, st, dIn,
+                       dst, (const int*)h->dPerm, n, nrhs, (int64_t)n, ldd, 1);
+  else
+    SWC_HIP(hipMemcpy2DAsync(dst, sizeof(double) * ldd, dIn, sizeof(double) * n,
+                             sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, st));
+  if (!on_device)
+    SWC_HIP(hipMemcpy2DAsync(X, sizeof(double) * ldx, h->dWork, sizeof(double) * n,
+                             sizeof(double) * n, nrhs, hipMemcpyDeviceToHost, st));
+  SWC_HIP(hipStreamSynchronize(st));
+  return SWC_OK;
+}
+
+// ---- refinement -------------------------------------------------------------
+
+// Per-column reduction (op 0 max|x|, op 1 x.y) of n x ncol device columns.
+int reduce_cols(swc_handle* h, const double* X, const double* Y, int ncol, int op, double* out) {
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks, ncol), dim3(kThreads), 0, st, X, Y, h->n,
+                     (int64_t)h->n, op, h->dPart);
+  hipLaunchKernelGGL(k_reduce_final, dim3(ncol), dim3(kThreads), 0, st, (const double*)h->dPart, op,
+                     h->dScal + 64);
+  SWC_HIP(hipMemcpyAsync(out, h->dScal + 64, sizeof(double) * ncol, hipMemcpyDeviceToHost, st));
+  SWC_HIP(hipStreamSynchronize(st));
+  return SWC_OK;
+}
+
+int put_coeffs(swc_handle* h, const std::vector<double>& c) {
+  SWC_HIP(hipMemcpyAsync(h->dScal + 64 + 2 * c.size(), c.data(), sizeof(double) * c.size(),
+                         hipMemcpyHostToDevice, h->stream));
+  return SWC_OK;
+}
+
+int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx);
+
+// Refines X for A X = B (both n x nrhs, factor ordering, ld n) until every
+// column's eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol.
+// method 0: classical refinement x += M^{-1} r; method 1: conjugate gradients
+// preconditioned with M = L L^T.  Starts from x = M^{-1} b.
+int refine_device(swc_handle* h, const double* Bt, double* Xt, int nrhs, double tol, int maxiter,
+                  int method, double* eta, int* iters) {
+  const int n = h->n;
+  const int64_t ld = n;
+  hipStream_t st = h->stream;
+  double* R = h->dRef + (int64_t)2 * n * nrhs;
+  double* Z = h->dRef + (int64_t)3 * n * nrhs;
+  double* Pv = h->dRef + (int64_t)4 * n * nrhs;
+  const dim3 g(std::min(grid_for(n), 2048), nrhs);
+  std::vector<double> bn(nrhs), xn(nrhs), rn(nrhs), rz(nrhs), pq(nrhs), co(nrhs);
+  std::vector<char> conv(nrhs, 0);
+  auto copy = [&](double* dst, const double* src) {
+    return hipMemcpyAsync(dst, src, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st);
+  };
+  auto evaluate = [&](double* Rbuf, bool& done) -> int {
+    hipLaunchKernelGGL(k_residual, g, dim3(kThreads), 0, st, (const int*)h->dRp,
+                       (const int*)h->dCi, (const double*)h->dAf, (const double*)Xt, Bt, Rbuf, n, ld);
+    if (int rc = reduce_cols(h, Rbuf, nullptr, nrhs, 0, rn.data())) return rc;
+    if (int rc = reduce_cols(h, Xt, nullptr, nrhs, 0, xn.data())) return rc;
+    done = true;
+    for (int c = 0; c < nrhs; ++c) {
+      const double den = h->normA * xn[c] + bn[c];
+      eta[c] = den > 0 ? rn[c] / den : 0.0;
+      conv[c] = eta[c] <= tol;
+      done = done && conv[c];
+    }
+    return SWC_OK;
+  };
+  if (int rc = reduce_cols(h, Bt, nullptr, nrhs, 0, bn.data())) return rc;
+  SWC_HIP(copy(Xt, Bt));
+  if (int rc = solve_device(h, Xt, nrhs, ld)) return rc;
+  int it = 0;
+  bool done = false;
+  if (int rc = evaluate(R, done)) return rc;
+  if (method == 0) {
+    for (; !done && it < maxiter; ++it) {
+      SWC_HIP(copy(Z, R));
+      if (int rc = solve_device(h, Z, nrhs, ld)) return rc;
+      for (int c = 0; c < nrhs; ++c) co[c] = conv[c] ? 0.0 : 1.0;
+      if (int rc = put_coeffs(h, co)) return rc;
+      hipLaunchKernelGGL(k_axpy_cols, g, dim3(kThreads), 0, st, Xt, (const double*)Z,
+                         (const double*)(h->dScal + 64 + 2 * nrhs), n, ld);
+      if (int rc = evaluate(R, done)) return rc;
+    }
+  } else {
+    if (!done) {
+      SWC_HIP(copy(Z, R));
+      if (int rc = solve_device(h, Z, nrhs, ld)) return rc;
+      SWC_HIP(copy(Pv, Z));
+      if (int rc = reduce_cols(h, R, Z, nrhs, 1, rz.data())) return rc;
+    }
+    for (; !done && it < maxiter; ++it) {
+      hipLaunchKernelGGL(k_spmv, g, dim3(kThreads), 0, st, (const int*)h->dRp, (const int*)h->dCi,
+                         (const double*)h->dAf, (const double*)Pv, Z, n, ld);
+      if (int rc = reduce_cols(h, Pv, Z, nrhs, 1, pq.data())) return rc;
+      for (int c = 0; c < nrhs; ++c) {
+        if (!conv[c] && !(pq[c] > 0))
+          return swc_set_error(SWC_ENOTPOS, "swc_refine: conjugate gradients broke down");
+        co[c] = conv[c] ? 0.0 : rz[c] / pq[c];
+      }
+      if (int rc = put_coeffs(h, co)) return rc;
+      hipLaunchKernelGGL(k_axpy_cols, g, dim3(kThreads), 0, st, Xt, (const double*)Pv,
+                         (const double*)(h->dScal + 64 + 2 * nrhs), n, ld);
+      for (int c = 0; c < nrhs; ++c) co[c] = -co[c];
+      if (int rc = put_coeffs(h, co)) return rc;
+      hipLaunchKernelGGL(k_axpy_cols, g, dim3(kThreads), 0, st, R, (const double*)Z,
+                         (const double*)(h->dScal + 64 + 2 * nrhs), n, ld);
+      if (int rc = evaluate(Z, done)) return rc;  // true residual, for eta only
+      if (done) {
+        ++it;
+        break;
+      }
+      SWC_HIP(copy(Z, R));
+      if (int rc = solve_device(h, Z, nrhs, ld)) return rc;
+      std::vector<double> rzn(nrhs);
+      if (int rc = reduce_cols(h, R, Z, nrhs, 1, rzn.data())) return rc;
+      for (int c = 0; c < nrhs; ++c) {
+        co[c] = (conv[c] || rz[c] == 0) ? 0.0 : rzn[c] / rz[c];
+        rz[c] = rzn[c];
+      }
+      if (int rc = put_coeffs(h, co)) return rc;
+      hipLaunchKernelGGL(k_xpby_cols, g, dim3(kThreads), 0, st, Pv, (const double*)Z,
+                         (const double*)(h->dScal + 64 + 2 * nrhs), n, ld);
+    }
+  }
+  SWC_HIP(hipGetLastError());
+  if (iters) *iters = it;
   return SWC_OK;
 }
 
@@ -693,6 +1053,36 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   h->nvals = a2src ? nvals : (int64_t)A2p[n];
   std::vector<int64_t> amap;
   if (int rc = build_amap(h, A2p, A2i, a2src, nvals, amap)) return rc;
+  // full CSR of P A P^T from its lower triangle, each entry mapped to its value
+  std::vector<int> frp(n + 1, 0), fci;
+  std::vector<int64_t> fvm;
+  for (int j = 0; j < n; ++j)
+    for (int q = A2p[j]; q < A2p[j + 1]; ++q) {
+      frp[A2i[q] + 1]++;
+      if (A2i[q] != j) frp[j + 1]++;
+    }
+  for (int i = 0; i < n; ++i) frp[i + 1] += frp[i];
+  fci.resize(frp[n]);
+  fvm.resize(frp[n]);
+  {
+    std::vector<int> fill(frp.begin(), frp.end() - 1);
+    // row i: first the lower-triangle entries (columns j <= i, ascending j) ...
+    for (int j = 0; j < n; ++j)
+      for (int q = A2p[j]; q < A2p[j + 1]; ++q) {
+        const int i = A2i[q];
+        fci[fill[i]] = j;
+        fvm[fill[i]++] = a2src ? a2src[q] : q;
+      }
+    // ... then the mirrored strictly-upper entries (columns > i, ascending)
+    for (int j = 0; j < n; ++j)
+      for (int q = A2p[j]; q < A2p[j + 1]; ++q) {
+        const int i = A2i[q];
+        if (i == j) continue;
+        fci[fill[j]] = i;
+        fvm[fill[j]++] = a2src ? a2src[q] : q;
+      }
+  }
+  h->nnz_full = frp[n];
   const Plan& P = h->plan;
   hipStream_t st = h->stream;
   int rc = SWC_OK;
@@ -709,6 +1099,11 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
   if (!rc) rc = upload(&h->dAmap, amap.data(), amap.size(), st);
   if (!rc) rc = upload(&h->dVals, (const double*)nullptr, (size_t)h->nvals, st);
+  if (!rc) rc = upload(&h->dRp, frp.data(), frp.size(), st);
+  if (!rc) rc = upload(&h->dCi, fci.data(), fci.size(), st);
+  if (!rc) rc = upload(&h->dVmap, fvm.data(), fvm.size(), st);
+  if (!rc) rc = upload(&h->dAf, (const double*)nullptr, fci.size(), st);
+  if (!rc) rc = upload(&h->dScal, (const double*)nullptr, 64, st);
   if (rc) return rc;
   SWC_HIP(hipStreamSynchronize(st));
   return SWC_OK;
@@ -788,57 +1183,67 @@ int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device) {
   if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_solve: not factored");
   if (h->last_info) return swc_set_error(SWC_ENOTPOS, "swc_solve: factorization failed");
   if (!B || nrhs <= 0 || ldb < h->n) return swc_set_error(SWC_EINVAL, "swc_solve: bad argument");
-  const int n = h->n;
-  hipStream_t st = h->stream;
   if (int rc = ensure_work(h, nrhs)) return rc;
-  double* dW = h->dWork;
-  double* dB = B;
-  if (!on_device) {
-    SWC_HIP(hipMemcpy2DAsync(dW, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n,
-                             nrhs, hipMemcpyHostToDevice, st));
-  }
-  const int64_t total = (int64_t)n * nrhs;
-  if (!h->perm.empty()) {
-    if (on_device) {
-      hipLaunchKernelGGL(k_permute, dim3(grid_for(total)), dim3(kThreads), 0, st,
-                         (const double*)dB, dW, (const int*)h->dPerm, n, nrhs, ldb, (int64_t)n, 0);
-    } else {
-      // staged copy is in dW; permute through dB-free path: copy then permute in place
-      // via a second buffer region is avoided by permuting from a temporary
-      double* tmp = nullptr;
-      SWC_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * total, st));
-      SWC_HIP(hipMemcpyAsync(tmp, dW, sizeof(double) * total, hipMemcpyDeviceToDevice, st));
-      hipLaunchKernelGGL(k_permute, dim3(grid_for(total)), dim3(kThreads), 0, st,
-                         (const double*)tmp, dW, (const int*)h->dPerm, n, nrhs, (int64_t)n,
-                         (int64_t)n, 0);
-      SWC_HIP(hipFreeAsync(tmp, st));
-    }
-  } else if (on_device) {
-    SWC_HIP(hipMemcpy2DAsync(dW, sizeof(double) * n, dB, sizeof(double) * ldb, sizeof(double) * n,
-                             nrhs, hipMemcpyDeviceToDevice, st));
-  }
-  if (int rc = solve_device(h, dW, nrhs, n)) return rc;
-  if (on_device) {
-    if (!h->perm.empty())
-      hipLaunchKernelGGL(k_permute, dim3(grid_for(total)), dim3(kThreads), 0, st,
-                         (const double*)dW, dB, (const int*)h->dPerm, n, nrhs, (int64_t)n, ldb, 1);
-    else
-      SWC_HIP(hipMemcpy2DAsync(dB, sizeof(double) * ldb, dW, sizeof(double) * n,
-                               sizeof(double) * n, nrhs, hipMemcpyDeviceToDevice, st));
-  } else {
-    std::vector<double> hostW((size_t)total);
-    SWC_HIP(hipMemcpyAsync(hostW.data(), dW, sizeof(double) * total, hipMemcpyDeviceToHost, st));
-    SWC_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < nrhs; ++r)
-      for (int k = 0; k < n; ++k) {
-        const int dst = h->perm.empty() ? k : h->perm[k];
-        B[(int64_t)r * ldb + dst] = hostW[(int64_t)r * n + k];
-      }
-  }
-  SWC_HIP(hipStreamSynchronize(st));
+  double* dY = h->dRef;
+  if (int rc = to_factor(h, B, on_device, ldb, dY, nrhs)) return rc;
+  if (int rc = solve_device(h, dY, nrhs, h->n)) return rc;
+  if (int rc = from_factor(h, dY, B, on_device, ldb, nrhs)) return rc;
   float ms = 0;
   SWC_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
   h->t_solve_ms = ms;
+  return SWC_OK;
+}
+
+int swc_refine(swc_handle* h, const double* B, double* X, int nrhs, int64_t ld, int on_device,
+               double tol, int maxiter, int method, double* eta, int* iters) {
+  if (int rc = check_handle(h)) return rc;
+  if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_refine: not factored");
+  if (h->last_info) return swc_set_error(SWC_ENOTPOS, "swc_refine: factorization failed");
+  if (!B || !X || nrhs <= 0 || ld < h->n || !eta)
+    return swc_set_error(SWC_EINVAL, "swc_refine: bad argument");
+  if (method < 0) method = h->factored_precision ? 1 : 0;
+  if (method > 1) return swc_set_error(SWC_EINVAL, "swc_refine: method must be -1, 0 or 1");
+  if (int rc = ensure_work(h, nrhs)) return rc;
+  double* Bt = h->dRef;
+  double* Xt = h->dRef + (int64_t)h->n * nrhs;
+  if (int rc = to_factor(h, B, on_device, ld, Bt, nrhs)) return rc;
+  if (int rc = refine_device(h, Bt, Xt, nrhs, tol, maxiter, method, eta, iters)) return rc;
+  return from_factor(h, Xt, X, on_device, ld, nrhs);
+}
+
+int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int64_t ld,
+                 int on_device, double* eta) {
+  if (int rc = check_handle(h)) return rc;
+  if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_residual: no values (factor first)");
+  if (!B || !X || nrhs <= 0 || ld < h->n || !eta)
+    return swc_set_error(SWC_EINVAL, "swc_residual: bad argument");
+  if (int rc = ensure_work(h, nrhs)) return rc;
+  const int n = h->n;
+  double* Bt = h->dRef;
+  double* Xt = h->dRef + (int64_t)n * nrhs;
+  double* R = h->dRef + (int64_t)2 * n * nrhs;
+  if (int rc = to_factor(h, B, on_device, ld, Bt, nrhs)) return rc;
+  if (int rc = to_factor(h, X, on_device, ld, Xt, nrhs)) return rc;
+  const dim3 g(std::min(grid_for(n), 2048), nrhs);
+  hipLaunchKernelGGL(k_residual, g, dim3(kThreads), 0, h->stream, (const int*)h->dRp,
+                     (const int*)h->dCi, (const double*)h->dAf, (const double*)Xt,
+                     (const double*)Bt, R, n, (int64_t)n);
+  std::vector<double> rn(nrhs), xn(nrhs), bn(nrhs);
+  if (int rc = reduce_cols(h, R, nullptr, nrhs, 0, rn.data())) return rc;
+  if (int rc = reduce_cols(h, Xt, nullptr, nrhs, 0, xn.data())) return rc;
+  if (int rc = reduce_cols(h, Bt, nullptr, nrhs, 0, bn.data())) return rc;
+  for (int c = 0; c < nrhs; ++c) {
+    const double den = h->normA * xn[c] + bn[c];
+    eta[c] = den > 0 ? rn[c] / den : 0.0;
+  }
+  return SWC_OK;
+}
+
+int swc_set_precision(swc_handle* h, int mode) {
+  if (int rc = check_handle(h)) return rc;
+  if (mode < 0 || mode > 2) return swc_set_error(SWC_EINVAL, "swc_set_precision: mode must be 0, 1 or 2");
+  h->precision = mode;
+  h->fp32_mask = mode == 0 ? 0 : (mode == 1 ? 1 : 3);
   return SWC_OK;
 }
 

@@ -91,7 +91,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     }
     t.ca = sr.ca;
     t.cb = sr.cb;
-    t.pad = 0;
+    t.kind = sr.d >= 0 ? 0 : 1;
     (void)st;
   };
 

@@ -52,7 +52,7 @@ struct Task {
   int rows_off;      // offset into Ls of the source's row 0
   int ra, rb;        // source rows landing in the tile's rows
   int ca, cb;        // source rows landing in the panel's columns
-  int pad;
+  int kind;          // 0: descendant supernode, 1: earlier panel of the same supernode
 };
 
 // Forward-solve gather into one panel from a descendant supernode d: rows
