@@ -130,8 +130,24 @@ int swc_refine(swc_handle* h, const double* B, double* X, int nrhs, int64_t ld, 
 /* eta[c] of a given X for A X = B (FP64, on the GPU). */
 int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int64_t ld,
                  int on_device, double* eta);
+/* ---- adjoint (no reference counterpart: the reference has no gradients) ----
+ * Selected inverse Z = A^{-1} on the pattern of L (L's BCSC layout, factor
+ * ordering), by the supernodal reverse sweep of the Cholesky adjoint; Z may be
+ * NULL to compute without copying out. */
+int swc_selected_inverse(swc_handle* h, double* Z, int on_device);
+/* logdet(A) = 2 sum_j log L_jj of the current factor. */
+int swc_logdet(swc_handle* h, double* logdet);
+/* grad[q] = w_q (alpha Z_ij - beta x_i x_j) for every input value q (an entry
+ * a_ij, i >= j, of the lower triangle, standing for a_ij and a_ji; w_q = 1 on the
+ * diagonal, 2 off it), i.e. the gradient of alpha logdet(A) + beta b^T A^{-1} b
+ * with respect to the stored values when x = A^{-1} b (original ordering). */
+int swc_gradient(swc_handle* h, const double* x, int x_on_device, double alpha, double beta,
+                 double* grad, int grad_on_device);
 /* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
 int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
+/* With graphs on (default), swc_factor and the solves replay launch sequences
+ * captured once per structure (hipGraph) instead of launching level by level. */
+int swc_set_graphs(swc_handle* h, int on);
 /* With profiling on, every kernel launch of swc_factor is bracketed by HIP events
  * on the handle's stream (for measurement runs; adds launch gaps). */
 int swc_set_profiling(swc_handle* h, int on);
@@ -142,12 +158,12 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[10] = {panels, tiles, tasks, levels,
- * solve gathers, xsize, TRSM tiles, solve row blocks, contribution-vector size,
- * most row blocks in a level}; swc_plan_export copies array `which` (0 panels,
+ * device, for tools and tests.  sizes[11] = {panels, tiles, tasks, levels,
+ * solve gathers, xsize, factor-list tiles, solve row blocks, contribution-vector
+ * size, most row blocks in a level, diagonal-block scratch doubles}; swc_plan_export copies array `which` (0 panels,
  * 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->TRSM-list offsets, 5 panels
  * by level, 6 level->panel offsets, 7 solve gathers, 8 panel->gather offsets,
- * 9 TRSM tile list, 10 solve row blocks, 11 panel(by level)->row-block offsets,
+ * 9 factor-list tiles, 10 solve row blocks, 11 panel(by level)->row-block offsets,
  * 12 level->row-block offsets) as the packed structs of
  * swcholesky_amd/csrc/plan.h. */
 typedef struct swc_plan swc_plan;

@@ -38,6 +38,10 @@ SIGNATURES = {
     "swc_solve": (c_int, [c_vp, c_vp, c_int, c_i64, c_int]),
     "swc_plan_stats": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_set_profiling": (c_int, [c_vp, c_int]),
+    "swc_set_graphs": (c_int, [c_vp, c_int]),
+    "swc_selected_inverse": (c_int, [c_vp, c_vp, c_int]),
+    "swc_logdet": (c_int, [c_vp, ctypes.POINTER(c_dbl)]),
+    "swc_gradient": (c_int, [c_vp, c_vp, c_int, c_dbl, c_dbl, c_vp, c_int]),
     "swc_set_precision": (c_int, [c_vp, c_int]),
     "swc_refine": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_int, c_dbl, c_int, c_int, c_vp, c_ip]),
     "swc_residual": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_int, c_vp]),

@@ -189,6 +189,34 @@ class Factor:
         check(_lib.lib().swc_residual(self._h, ptr(B), ptr(X), B.shape[1], self.n, 0, ptr(eta)))
         return eta
 
+    def selected_inverse(self):
+        """Z = A^{-1} on the pattern of L (L's BCSC layout, factor ordering)."""
+        out = np.zeros(self.bcsc.xsize)
+        check(_lib.lib().swc_selected_inverse(self._h, ptr(out), 0))
+        return out
+
+    def logdet(self):
+        v = ctypes.c_double()
+        check(_lib.lib().swc_logdet(self._h, ctypes.byref(v)))
+        return v.value
+
+    def gradient(self, x=None, alpha=1.0, beta=0.0, out=None):
+        """Gradient of alpha logdet(A) + beta b^T A^{-1} b w.r.t. the stored values
+        (x = A^{-1} b).  numpy in -> numpy out; torch cuda tensors in -> `out`."""
+        lib = _lib.lib()
+        if out is not None and not isinstance(out, np.ndarray):
+            xp, xdev = _dptr(x) if x is not None else (None, 1)
+            gp, gdev = _dptr(out)
+            check(lib.swc_gradient(self._h, xp, xdev, float(alpha), float(beta), gp, gdev))
+            return out
+        g = np.zeros(self.bcsc.nnz)
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        check(lib.swc_gradient(self._h, ptr(xa), 0, float(alpha), float(beta), ptr(g), 0))
+        return g
+
+    def set_graphs(self, on=True):
+        check(_lib.lib().swc_set_graphs(self._h, int(on)))
+
     def set_profiling(self, on=True):
         check(_lib.lib().swc_set_profiling(self._h, int(on)))
 
@@ -214,7 +242,7 @@ PANEL_DT = np.dtype([("vbase", np.int64), ("voff", np.int64), ("ld", np.int32),
                      ("sw", np.int32)])
 TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("task_begin", np.int32), ("task_end", np.int32), ("kind", np.int32),
-                    ("pad0", np.int32), ("pad1", np.int32)])
+                    ("slot", np.int32), ("pad", np.int32)])
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
                     ("ca", np.int32), ("cb", np.int32), ("kind", np.int32)])
@@ -233,10 +261,10 @@ def build_plan(bcsc: BCSC):
     check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
                              ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
     try:
-        sz = np.zeros(10, dtype=np.int64)
+        sz = np.zeros(11, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm, nsblk, vsize, maxlb = (
+        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm, nsblk, vsize, maxlb, dscr = (
             int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
@@ -252,7 +280,7 @@ def build_plan(bcsc: BCSC):
             check(lib.swc_plan_export(h, which, ptr(a)))
             out[name] = a
         out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1],
-                   vsize=vsize, max_lvl_sblk=maxlb)
+                   vsize=vsize, max_lvl_sblk=maxlb, diag_scratch=dscr)
         return out
     finally:
         lib.swc_plan_free(h)

@@ -95,19 +95,27 @@ __device__ inline void task_mfma(const double* __restrict__ src, int ld, int k, 
   const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;  // wave-uniform
   const double* pa = src + ra;
   const double* pb = src + ca;
-  for (int kk = 0; kk < k; kk += 4) {
-    const int kc = kk + lk;
-    const bool kok = kc < k;
-    const int64_t off = (int64_t)kc * ld;
-    const double a0 = (a0ok && kok) ? pa[off + r0] : 0.0;
-    const double a1 = (a1ok && kok) ? pa[off + r1] : 0.0;
-    const double b0 = (b0ok && kok) ? pb[off + q0] : 0.0;
-    const double b1 = (b1ok && kok) ? pb[off + q1] : 0.0;
-    c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0][0], 0, 0, 0);
-    if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[0][1], 0, 0, 0);
-    if (blk_a1) {
-      c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[1][0], 0, 0, 0);
-      if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[1][1], 0, 0, 0);
+  // 16 k per iteration: all operand loads are issued before the 16 MFMAs
+  for (int kk = 0; kk < k; kk += 16) {
+    double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kc = kk + 4 * u + lk;
+      const bool kok = kc < k;
+      const int64_t off = (int64_t)kc * ld;
+      a0[u] = (a0ok && kok) ? pa[off + r0] : 0.0;
+      a1[u] = (a1ok && kok) ? pa[off + r1] : 0.0;
+      b0[u] = (b0ok && kok) ? pb[off + q0] : 0.0;
+      b1[u] = (b1ok && kok) ? pb[off + q1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], c[0][0], 0, 0, 0);
+      if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], c[0][1], 0, 0, 0);
+      if (blk_a1) {
+        c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], c[1][0], 0, 0, 0);
+        if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], c[1][1], 0, 0, 0);
+      }
     }
   }
 }
@@ -132,37 +140,42 @@ __device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int
   const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;
   const double* pa = src + ra;
   const double* pb = src + ca;
-  for (int kk = 0; kk < k; kk += 4) {
-    const int kc = kk + lk;
-    const bool kok = kc < k;
-    const int64_t off = (int64_t)kc * ld;
-    const float a0 = (a0ok && kok) ? (float)pa[off + r0] : 0.f;
-    const float a1 = (a1ok && kok) ? (float)pa[off + r1] : 0.f;
-    const float b0 = (b0ok && kok) ? (float)pb[off + q0] : 0.f;
-    const float b1 = (b1ok && kok) ? (float)pb[off + q1] : 0.f;
-    c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, c[0][0], 0, 0, 0);
-    if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, c[0][1], 0, 0, 0);
-    if (blk_a1) {
-      c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, c[1][0], 0, 0, 0);
-      if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[1][1], 0, 0, 0);
+  for (int kk = 0; kk < k; kk += 16) {
+    float a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kc = kk + 4 * u + lk;
+      const bool kok = kc < k;
+      const int64_t off = (int64_t)kc * ld;
+      a0[u] = (a0ok && kok) ? (float)pa[off + r0] : 0.f;
+      a1[u] = (a1ok && kok) ? (float)pa[off + r1] : 0.f;
+      b0[u] = (b0ok && kok) ? (float)pb[off + q0] : 0.f;
+      b1[u] = (b1ok && kok) ? (float)pb[off + q1] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], c[0][0], 0, 0, 0);
+      if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], c[0][1], 0, 0, 0);
+      if (blk_a1) {
+        c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], c[1][0], 0, 0, 0);
+        if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], c[1][1], 0, 0, 0);
+      }
     }
   }
 }
 
-// Tile kernel: accumulate all source updates of one tile in LDS (in task order),
-// then, for a diagonal tile, factor the panel's diagonal block and solve the
-// tile's remaining rows against it.  info_out receives min(first failing
-// global column + 1).
+// Tile kernel: accumulate all source updates of one tile in LDS (in task order)
+// and write the tile back.  The factorization of panels completed by this
+// level follows in k_panel_factor.
 __global__ void __launch_bounds__(kThreads)
-k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
-                     const Panel* __restrict__ panels, const Tile* __restrict__ tiles,
-                     const Task* __restrict__ tasks, int tile0, int* __restrict__ info_out,
-                     int fp32_mask) {
+k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
+              const Panel* __restrict__ panels, const Tile* __restrict__ tiles,
+              const Task* __restrict__ tasks, int tile0, int fp32_mask,
+              double* __restrict__ dscratch) {
   __shared__ double acc[swc::PW * kLdsLd];
   __shared__ int trow[swc::TM];
   __shared__ int pr[swc::TM];
   __shared__ int pc[swc::TM];
-  __shared__ int bad;
   const Tile tl = tiles[tile0 + blockIdx.x];
   const Panel pn = panels[tl.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -173,7 +186,6 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
     const int i = e % nr, j = e / nr;
     acc[j * kLdsLd + i] = gtile[(int64_t)j * pn.ld + i];
   }
-  if (tid == 0) bad = 0;
   __syncthreads();
   for (int t = tl.task_begin; t < tl.task_end; ++t) {
     const Task tk = tasks[t];
@@ -231,74 +243,98 @@ k_tile_update_factor(double* __restrict__ L, const int* __restrict__ Ls,
     }
     __syncthreads();
   }
-  if (tl.kind == 0) {
-    // right-looking LL^T of the nr x nc trapezoid held in LDS
-    for (int j = 0; j < nc; ++j) {
-      if (tid == 0) {
-        const double d = acc[j * kLdsLd + j];
-        if (!(d > 0.0)) {
-          if (!bad) atomicMin(info_out, pn.c0 + j + 1);
-          bad = 1;
-          acc[j * kLdsLd + j] = 1.0;  // keep going without NaNs; info reports it
-        } else {
-          acc[j * kLdsLd + j] = sqrt(d);
-        }
-      }
-      __syncthreads();
-      const double piv = acc[j * kLdsLd + j];
-      for (int i = j + 1 + tid; i < nr; i += kThreads) acc[j * kLdsLd + i] /= piv;
-      __syncthreads();
-      const int cols = nc - j - 1;
-      for (int e = tid; e < cols * nr; e += kThreads) {
-        const int c = j + 1 + e / nr, i = e % nr;
-        if (i >= c) acc[c * kLdsLd + i] -= acc[j * kLdsLd + i] * acc[j * kLdsLd + c];
-      }
-      __syncthreads();
-    }
-  }
   for (int e = tid; e < nr * nc; e += kThreads) {
     const int i = e % nr, j = e / nr;
     if (tl.kind == 0 && i < j) continue;  // keep the strict upper part untouched (zero)
     gtile[(int64_t)j * pn.ld + i] = acc[j * kLdsLd + i];
   }
+  if (tl.kind == 0 && tl.slot >= 0) {
+    // this level factors the panel: hand the updated diagonal block to every
+    // factor-kernel tile through a copy that nobody overwrites in that launch
+    double* dcopy = dscratch + tl.slot;
+    for (int e = tid; e < nc * nc; e += kThreads) {
+      const int i = e % nc, j = e / nc;
+      dcopy[e] = i >= j ? acc[j * kLdsLd + i] : 0.0;
+    }
+  }
 }
 
-// Off-diagonal tile TRSM: B <- B L11^{-T}, L11 the panel's factored diagonal block.
-__global__ void __launch_bounds__(kThreads)
-k_tile_trsm(double* __restrict__ L, const Panel* __restrict__ panels,
-            const Tile* __restrict__ tiles, const int* __restrict__ list, int list0) {
-  __shared__ double acc[swc::PW * kLdsLd];
+// Panel factorization, one wave per tile of a panel completed at this level:
+// POTRF of the panel's nc x nc diagonal block (dpotrf 'L',
+// sw_parallel_PB_Cholesky_05.c:430) in registers -- lane i holds row i, column
+// j's multipliers are broadcast by shuffles -- then B <- B L11^{-T} for the
+// tile's rows below the diagonal block (dtrsm 'R','L','T','N', .c:461-467),
+// lane i substituting its own row against L11 staged in LDS.  Every tile of a
+// panel computes the same L11 from the same inputs (bitwise identical), so no
+// tile waits for another; the diagonal tile stores it and reports info.
+constexpr int kFactThreads = 64;
+
+__global__ void __launch_bounds__(kFactThreads)
+k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
+               const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
+               const double* __restrict__ dscratch, int* __restrict__ info_out) {
   __shared__ double l11[swc::PW * kLdsLd];
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int nr = tl.nr, nc = pn.ncol;
-  double* gtile = L + pn.vbase + tl.r0;
-  const double* gdiag = L + pn.vbase;
-  for (int e = tid; e < nr * nc; e += kThreads) {
-    const int i = e % nr, j = e / nr;
-    acc[j * kLdsLd + i] = gtile[(int64_t)j * pn.ld + i];
-  }
-  for (int e = tid; e < nc * nc; e += kThreads) {
-    const int i = e % nc, j = e / nc;
-    l11[j * kLdsLd + i] = gdiag[(int64_t)j * pn.ld + i];
+  const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
+  __shared__ double colj[swc::PW];  // column j of L11 during the factorization
+  double x[swc::PW];
+#pragma unroll
+  for (int c = 0; c < swc::PW; ++c) x[c] = (lane < nc && c <= lane) ? gdiag[c * nc + lane] : 0.0;
+  int badcol = 0;
+#pragma unroll
+  for (int j = 0; j < swc::PW; ++j) {
+    if (j < nc) {
+      // the pivot lives in lane j; publish it, then the scaled column, via LDS
+      if (lane == j) colj[j] = x[j];
+      __syncthreads();
+      const double d = colj[j];
+      const bool ok = d > 0.0;
+      if (!ok && badcol == 0) badcol = j + 1;
+      const double sq = ok ? sqrt(d) : 1.0;  // keep going without NaNs; info reports it
+      if (lane == j) x[j] = sq;
+      else if (lane > j) x[j] /= sq;
+      __syncthreads();
+      if (lane < nc) colj[lane] = x[j];
+      __syncthreads();
+      const double lij = x[j];
+#pragma unroll
+      for (int c = j + 1; c < swc::PW; ++c)
+        if (c < nc && lane >= c) x[c] -= lij * colj[c];
+    }
   }
   __syncthreads();
-  for (int j = 0; j < nc; ++j) {
-    const double inv = 1.0 / l11[j * kLdsLd + j];
-    for (int i = tid; i < nr; i += kThreads) acc[j * kLdsLd + i] *= inv;
-    __syncthreads();
-    const int cols = nc - j - 1;
-    for (int e = tid; e < cols * nr; e += kThreads) {
-      const int c = j + 1 + e / nr, i = e % nr;
-      acc[c * kLdsLd + i] -= acc[j * kLdsLd + i] * l11[j * kLdsLd + c];
+  // stage L11 (row `lane` of it) in LDS
+#pragma unroll
+  for (int c = 0; c < swc::PW; ++c)
+    if (lane < nc && c < nc) l11[c * kLdsLd + lane] = c <= lane ? x[c] : 0.0;
+  double* gtile = L + pn.vbase + tl.r0;
+  if (tl.kind == 0) {
+    if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
+#pragma unroll
+    for (int c = 0; c < swc::PW; ++c)
+      if (lane < nc && c <= lane) gtile[(int64_t)c * pn.ld + lane] = x[c];
+  }
+  // rows of this tile below the diagonal block: tile rows [first, nr)
+  const int first = tl.kind == 0 ? nc : 0;
+  const bool mine = lane >= first && lane < nr;
+#pragma unroll
+  for (int c = 0; c < swc::PW; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < swc::PW; ++j) {
+    if (j < nc) {
+      x[j] *= 1.0 / l11[j * kLdsLd + j];
+#pragma unroll
+      for (int c = j + 1; c < swc::PW; ++c)
+        if (c < nc) x[c] -= x[j] * l11[j * kLdsLd + c];
     }
-    __syncthreads();
   }
-  for (int e = tid; e < nr * nc; e += kThreads) {
-    const int i = e % nr, j = e / nr;
-    gtile[(int64_t)j * pn.ld + i] = acc[j * kLdsLd + i];
-  }
+#pragma unroll
+  for (int c = 0; c < swc::PW; ++c)
+    if (mine && c < nc) gtile[(int64_t)c * pn.ld + lane] = x[c];
 }
 
 // ---- triangular solves --------------------------------------------------
@@ -499,12 +535,12 @@ __global__ void k_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
   }
 }
 
-// ||A||_inf = max_i sum_j |a_ij|, written to out[0] (single block, deterministic).
+// Partial maxima of the row sums of |A| (||A||_inf), one per block of a fixed grid.
 __global__ void k_norm_inf(const int* __restrict__ rp, const double* __restrict__ av, int n,
-                           double* __restrict__ out) {
+                           double* __restrict__ part) {
   __shared__ double red[kThreads];
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += kThreads) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += kRedBlocks * kThreads) {
     double t = 0.0;
     for (int e = rp[i]; e < rp[i + 1]; ++e) t += fabs(av[e]);
     m = fmax(m, t);
@@ -515,7 +551,7 @@ __global__ void k_norm_inf(const int* __restrict__ rp, const double* __restrict_
     if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // Per-column partial reductions over a fixed grid (deterministic): op 0 max|x|,
@@ -577,6 +613,139 @@ __global__ void k_xpby_cols(double* __restrict__ P, const double* __restrict__ Z
     P[c * ld + i] = fma(b, P[c * ld + i], Z[c * ld + i]);
 }
 
+// ---- adjoint: selected inverse on the pattern of L ------------------------
+// Z = A^{-1} restricted to the pattern of L (in L's BCSC layout), by the
+// supernodal Takahashi recurrences, supernodes of one level per launch, root
+// level first.  For supernode J with L = [L11; L21] (rows R below):
+//   T   = L11^{-1}
+//   U   = L21 T                       (recomputed from L, not stored by the factor)
+//   Z21 = -Z(R,R) U                   (Z(R,R) gathered from the ancestors' Z)
+//   Z11 = T^T T - U^T Z21
+// This is the reverse sweep of the blocked Cholesky adjoint for
+// d logdet(A) / dA = A^{-1} (no reference counterpart).
+
+struct SelinvNode {
+  int64_t vbase;    // value offset of the supernode (row 0, column 0)
+  int64_t scratch;  // offset of its T | U | Zrr scratch
+  int w, nrow;      // width, rows
+  int c0;           // first column
+  int rows_off;     // offset into Ls
+};
+
+__device__ inline int lower_pos(const int* a, int len, int v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_selinv(const double* __restrict__ L, double* __restrict__ Z, const int* __restrict__ Ls,
+         const int* __restrict__ col2sup, const int64_t* __restrict__ spx,
+         const int* __restrict__ ssuper, const int64_t* __restrict__ spi,
+         const SelinvNode* __restrict__ nodes, int node0, double* __restrict__ arena) {
+  const SelinvNode nd = nodes[node0 + blockIdx.x];
+  const int tid = threadIdx.x;
+  const int W = nd.w, R = nd.nrow, m = R - W;
+  const double* L11 = L + nd.vbase;  // ld R
+  const double* L21 = L11 + W;
+  double* T = arena + nd.scratch;    // W x W, ld W (column-major, lower)
+  double* U = T + (int64_t)W * W;    // m x W, ld m
+  double* Zrr = U + (int64_t)m * W;  // m x m, ld m (full symmetric)
+  double* Zs = Z + nd.vbase;
+  const int* rows = Ls + nd.rows_off;
+  // T = L11^{-1}: column j by forward substitution (one thread per column)
+  for (int j = tid; j < W; j += kThreads) {
+    T[(int64_t)j * W + j] = 1.0 / L11[(int64_t)j * R + j];
+    for (int i = 0; i < j; ++i) T[(int64_t)j * W + i] = 0.0;
+    for (int i = j + 1; i < W; ++i) {
+      double sum = 0.0;
+      for (int k = j; k < i; ++k) sum = fma(L11[(int64_t)k * R + i], T[(int64_t)j * W + k], sum);
+      T[(int64_t)j * W + i] = -sum / L11[(int64_t)i * R + i];
+    }
+  }
+  __syncthreads();
+  // U = L21 T
+  for (int64_t e = tid; e < (int64_t)m * W; e += kThreads) {
+    const int i = (int)(e % m), c = (int)(e / m);
+    double sum = 0.0;
+    for (int k = c; k < W; ++k) sum = fma(L21[(int64_t)k * R + i], T[(int64_t)c * W + k], sum);
+    U[(int64_t)c * m + i] = sum;
+  }
+  // Zrr = Z(R, R), both triangles, from the ancestors' selected inverse
+  for (int64_t e = tid; e < (int64_t)m * m; e += kThreads) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    if (i < j) continue;
+    const int ri = rows[W + i], rj = rows[W + j];
+    const int K = col2sup[rj];
+    const int rk0 = (int)spi[K];
+    const int lenK = (int)(spi[K + 1] - spi[K]);
+    const int pos = lower_pos(Ls + rk0, lenK, ri);
+    const double z = Z[spx[K] + (int64_t)(rj - ssuper[K]) * lenK + pos];
+    Zrr[(int64_t)j * m + i] = z;
+    Zrr[(int64_t)i * m + j] = z;
+  }
+  __syncthreads();
+  // Z21 = -Zrr U, written into Z (rows W.. of the supernode)
+  for (int64_t e = tid; e < (int64_t)m * W; e += kThreads) {
+    const int i = (int)(e % m), c = (int)(e / m);
+    double sum = 0.0;
+    for (int j = 0; j < m; ++j) sum = fma(Zrr[(int64_t)j * m + i], U[(int64_t)c * m + j], sum);
+    Zs[(int64_t)c * R + W + i] = -sum;
+  }
+  __syncthreads();
+  // Z11 = T^T T - U^T Z21 (lower triangle; strict upper stays zero)
+  for (int64_t e = tid; e < (int64_t)W * W; e += kThreads) {
+    const int i = (int)(e % W), c = (int)(e / W);
+    if (i < c) {
+      Zs[(int64_t)c * R + i] = 0.0;
+      continue;
+    }
+    double sum = 0.0;
+    for (int k = i; k < W; ++k) sum = fma(T[(int64_t)i * W + k], T[(int64_t)c * W + k], sum);
+    double sub = 0.0;
+    for (int j = 0; j < m; ++j) sub = fma(U[(int64_t)i * m + j], Zs[(int64_t)c * R + W + j], sub);
+    Zs[(int64_t)c * R + i] = sum - sub;
+  }
+}
+
+// logdet partials: sum of log L_jj over the diagonal entries (fixed grid).
+__global__ void k_logdet_partial(const double* __restrict__ L, const int64_t* __restrict__ diag,
+                                 int n, double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += kRedBlocks * kThreads)
+    acc += log(L[diag[j]]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Gradient w.r.t. the input values of A (each stored lower entry a_q stands for
+// a_ij and a_ji):  g_q = w_q (alpha Z_ij - beta x_i x_j),  w_q = 1 on the
+// diagonal, 2 off it; Z at zpos[q] (offset in L's layout), x in factor order.
+__global__ void k_grad_values(const double* __restrict__ Zv, const int64_t* __restrict__ zpos,
+                              const int* __restrict__ qi, const int* __restrict__ qj,
+                              const double* __restrict__ x, double alpha, double beta,
+                              int64_t nvals, double* __restrict__ g) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nvals;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int i = qi[q], j = qj[q];
+    const double w = i == j ? 1.0 : 2.0;
+    double v = 0.0;
+    if (alpha != 0.0) v += alpha * Zv[zpos[q]];
+    if (beta != 0.0) v -= beta * x[i] * x[j];
+    g[q] = w * v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -590,6 +759,7 @@ struct swc_handle {
   std::vector<int> super, Ls;
   std::vector<int64_t> pi, px;
   std::vector<int> perm;  // perm[new] = old, empty if not set
+  std::vector<int> pruneptr, pruneset;
   // device buffers
   double* dL = nullptr;
   int* dLs = nullptr;
@@ -604,6 +774,7 @@ struct swc_handle {
   double* dPb = nullptr;  // backward partial sums, max_lvl_sblk * PW * work_cols
   int* dOrder = nullptr;
   int* dTrsm = nullptr;
+  double* dDiag = nullptr;  // per-level copies of the diagonal blocks being factored
   int64_t* dAmap = nullptr;
   int* dPerm = nullptr;
   double* dVals = nullptr;  // staging for host values
@@ -617,14 +788,39 @@ struct swc_handle {
   double* dAf = nullptr;
   int64_t nnz_full = 0;
   double* dScal = nullptr;   // small device scalars (norms, reductions)
+  double* dNormPart = nullptr;  // kRedBlocks partial maxima of ||A||_inf
   double* dPart = nullptr;   // reduction partials, kRedBlocks * work_cols
   double* dRef = nullptr;    // refinement vectors, 5 * n * work_cols
   double normA = 0;
+  // adjoint (selected inverse) state, built on first use
+  double* dZ = nullptr;          // selected inverse, xsize
+  double* dArena = nullptr;      // per-level T | U | Zrr scratch
+  SelinvNode* dNodes = nullptr;
+  int* dCol2sup = nullptr;
+  int64_t* dSpx = nullptr;
+  int* dSsuper = nullptr;
+  int64_t* dSpi = nullptr;
+  int64_t* dDiagPos = nullptr;   // offset of L(j,j), j in factor order
+  int* dQi = nullptr;            // factor-order row / column of every input value
+  int* dQj = nullptr;
+  std::vector<int> sel_lvl;      // level -> node offsets (root level first)
+  bool zvalid = false;
   int precision = 0;  // 0 FP64; 1 FP32 descendant updates; 2 FP32 for every update
   int fp32_mask = 0;  // bit k: tasks of kind k use FP32 MFMA
   int factored_precision = 0;
   bool factored = false;
   int last_info = 0;
+  // captured launch sequences (hipGraph): factor per precision mode, solves per
+  // (buffer, nrhs, ld); rebuilt when the structure changes
+  bool use_graphs = true;
+  hipGraphExec_t fgraph[3] = {nullptr, nullptr, nullptr};
+  struct SolveGraph {
+    const double* x;
+    int nrhs;
+    int64_t ld;
+    hipGraphExec_t exec;
+  };
+  std::vector<SolveGraph> sgraphs;
   // timing (device events on `stream`)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool profile = false;
@@ -633,17 +829,29 @@ struct swc_handle {
   int n_tile = 0, n_trsm = 0;
 
   ~swc_handle() { release(); }
+  void drop_graphs() {
+    for (auto& g : fgraph)
+      if (g) hipGraphExecDestroy(g);
+    for (auto& g : fgraph) g = nullptr;
+    for (auto& g : sgraphs) hipGraphExecDestroy(g.exec);
+    sgraphs.clear();
+  }
   void release() {
+    drop_graphs();
     hipFree(dL); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
-    hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dAmap); hipFree(dPerm);
+    hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
-    hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart);
+    hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart); hipFree(dNormPart);
     hipFree(dRef);
+    hipFree(dZ); hipFree(dArena); hipFree(dNodes); hipFree(dCol2sup); hipFree(dSpx);
+    hipFree(dSsuper); hipFree(dSpi); hipFree(dDiagPos); hipFree(dQi); hipFree(dQj);
     dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
-    dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr;
+    dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     dRef = nullptr;
+    dZ = nullptr; dArena = nullptr; dNodes = nullptr; dCol2sup = nullptr; dSpx = nullptr;
+    dSsuper = nullptr; dSpi = nullptr; dDiagPos = nullptr; dQi = nullptr; dQj = nullptr;
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     for (auto& e : pev) hipEventDestroy(e);
@@ -701,6 +909,8 @@ int build_amap(const swc_handle* h, const int* a2p, const int* a2i, const int64_
 
 int ensure_work(swc_handle* h, int cols) {
   if (h->work_cols >= cols) return SWC_OK;
+  for (auto& g : h->sgraphs) hipGraphExecDestroy(g.exec);  // they captured the old buffers
+  h->sgraphs.clear();
   hipFree(h->dWork);
   hipFree(h->dV);
   hipFree(h->dPb);
@@ -721,43 +931,70 @@ int ensure_work(swc_handle* h, int cols) {
   return SWC_OK;
 }
 
-// Numeric factorization from values already on the device (in upload order).
-int factor_device(swc_handle* h, const double* dvals) {
+// Enqueues the whole numeric factorization of the staged values h->dVals.
+void enqueue_factor(swc_handle* h, bool profile) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
-  if (h->profile && h->pev.size() < (size_t)4 * P.nlevel) {
-    for (auto& e : h->pev) hipEventDestroy(e);
-    h->pev.assign((size_t)4 * P.nlevel, nullptr);
-    for (auto& e : h->pev) SWC_HIP(hipEventCreate(&e));
-  }
-  SWC_HIP(hipEventRecord(h->ev[0], st));
+  const double* dvals = h->dVals;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(P.xsize)), dim3(kThreads), 0, st, h->dL, P.xsize, 0.0);
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(h->nvals)), dim3(kThreads), 0, st, h->dL,
                      (const int64_t*)h->dAmap, dvals, h->nvals);
   hipLaunchKernelGGL(k_gather, dim3(grid_for(h->nnz_full)), dim3(kThreads), 0, st, h->dAf, dvals,
                      (const int64_t*)h->dVmap, h->nnz_full);
-  hipLaunchKernelGGL(k_norm_inf, dim3(1), dim3(kThreads), 0, st, (const int*)h->dRp,
-                     (const double*)h->dAf, h->n, h->dScal);
-  const int big = 0x7fffffff;
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, big);
+  hipLaunchKernelGGL(k_norm_inf, dim3(kRedBlocks), dim3(kThreads), 0, st, (const int*)h->dRp,
+                     (const double*)h->dAf, h->n, h->dNormPart);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, (const double*)h->dNormPart, 0,
+                     h->dScal);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, 0x7fffffff);
   for (int l = 0; l < P.nlevel; ++l) {
     const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0;
     const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
-    hipEvent_t* pe = h->profile ? &h->pev[(size_t)4 * l] : nullptr;
-    if (pe) SWC_HIP(hipEventRecord(pe[0], st));
+    hipEvent_t* pe = profile ? &h->pev[(size_t)4 * l] : nullptr;
+    if (pe) hipEventRecord(pe[0], st);
     if (nt > 0)
-      hipLaunchKernelGGL(k_tile_update_factor, dim3(nt), dim3(kThreads), 0, st, h->dL,
+      hipLaunchKernelGGL(k_tile_update, dim3(nt), dim3(kThreads), 0, st, h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0, h->dInfo, h->fp32_mask);
-    if (pe) SWC_HIP(hipEventRecord(pe[1], st));
+                         (const Task*)h->dTasks, t0, h->fp32_mask, h->dDiag);
+    if (pe) hipEventRecord(pe[1], st);
     if (ns > 0)
-      hipLaunchKernelGGL(k_tile_trsm, dim3(ns), dim3(kThreads), 0, st, h->dL,
+      hipLaunchKernelGGL(k_panel_factor, dim3(ns), dim3(kFactThreads), 0, st, h->dL,
                          (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const int*)h->dTrsm, s0);
-    if (pe) SWC_HIP(hipEventRecord(pe[2], st));
+                         (const int*)h->dTrsm, s0, (const double*)h->dDiag, h->dInfo);
+    if (pe) hipEventRecord(pe[2], st);
   }
+}
+
+// Numeric factorization from values on the device (in upload order).
+int factor_device(swc_handle* h, const double* dvals) {
+  hipStream_t st = h->stream;
+  const Plan& P = h->plan;
+  if (dvals != h->dVals)
+    SWC_HIP(hipMemcpyAsync(h->dVals, dvals, sizeof(double) * h->nvals, hipMemcpyDeviceToDevice, st));
+  if (h->profile && h->pev.size() < (size_t)4 * P.nlevel) {
+    for (auto& e : h->pev) hipEventDestroy(e);
+    h->pev.assign((size_t)4 * P.nlevel, nullptr);
+    for (auto& e : h->pev) SWC_HIP(hipEventCreate(&e));
+  }
+  const bool graph = h->use_graphs && !h->profile;
+  hipGraphExec_t& gx = h->fgraph[h->precision];
+  if (graph && !gx) {
+    hipGraph_t g = nullptr;
+    SWC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_factor(h, false);
+    SWC_HIP(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      gx = nullptr;
+      return swc_set_error(SWC_EDEVICE, std::string("factor graph: ") + hipGetErrorString(e));
+    }
+  }
+  SWC_HIP(hipEventRecord(h->ev[0], st));
+  if (graph) SWC_HIP(hipGraphLaunch(gx, st));
+  else enqueue_factor(h, h->profile);
   SWC_HIP(hipEventRecord(h->ev[1], st));
   SWC_HIP(hipGetLastError());
+  const int big = 0x7fffffff;
   int info = 0;
   SWC_HIP(hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st));
   SWC_HIP(hipMemcpyAsync(&h->normA, h->dScal, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -787,17 +1024,16 @@ int factor_device(swc_handle* h, const double* dvals) {
   h->last_info = info == big ? 0 : info;
   h->factored = true;
   h->factored_precision = h->precision;
+  h->zvalid = false;
   return SWC_OK;
 }
 
-// In-place solve of L L^T X = B on device columns X (factor ordering); needs
-// ensure_work(h, nrhs).
-int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+// Enqueues the forward and backward sweeps on device columns X (factor ordering).
+void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
   const int64_t pld = (int64_t)std::max(P.max_lvl_sblk, 1) * swc::PW;
-  SWC_HIP(hipEventRecord(h->ev[2], st));
   hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
   for (int l = 0; l < P.nlevel; ++l) {
     const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
@@ -825,6 +1061,34 @@ int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
                          (const int*)h->dSblkPtr, b0, (const double*)h->dPb, pld, dX, nrhs, ldx);
   }
+}
+
+// In-place solve of L L^T X = B on device columns X (factor ordering); needs
+// ensure_work(h, nrhs).
+int solve_device(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+  hipStream_t st = h->stream;
+  hipGraphExec_t gx = nullptr;
+  if (h->use_graphs) {
+    for (auto& g : h->sgraphs)
+      if (g.x == dX && g.nrhs == nrhs && g.ld == ldx) gx = g.exec;
+    if (!gx) {
+      if (h->sgraphs.size() >= 8) {
+        hipGraphExecDestroy(h->sgraphs.front().exec);
+        h->sgraphs.erase(h->sgraphs.begin());
+      }
+      hipGraph_t g = nullptr;
+      SWC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue_solve(h, dX, nrhs, ldx);
+      SWC_HIP(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return swc_set_error(SWC_EDEVICE, std::string("solve graph: ") + hipGetErrorString(e));
+      h->sgraphs.push_back({dX, nrhs, ldx, gx});
+    }
+  }
+  SWC_HIP(hipEventRecord(h->ev[2], st));
+  if (gx) SWC_HIP(hipGraphLaunch(gx, st));
+  else enqueue_solve(h, dX, nrhs, ldx);
   SWC_HIP(hipEventRecord(h->ev[3], st));
   SWC_HIP(hipGetLastError());
   return SWC_OK;
@@ -988,6 +1252,74 @@ int refine_device(swc_handle* h, const double* Bt, double* Xt, int nrhs, double 
   return SWC_OK;
 }
 
+// ---- adjoint ---------------------------------------------------------------
+
+int selinv_setup(swc_handle* h) {
+  if (h->dNodes) return SWC_OK;
+  const int ns = h->nsuper;
+  hipStream_t st = h->stream;
+  // supernode levels of the descendant DAG; the sweep runs them top-down
+  std::vector<int> lev(ns, 0), col2sup(h->n);
+  int nlev = 0;
+  std::vector<int> pp(h->pruneptr), ps(h->pruneset);
+  for (int s = 0; s < ns; ++s) {
+    for (int t = pp[s]; t < pp[s + 1]; ++t) lev[s] = std::max(lev[s], lev[ps[t]] + 1);
+    nlev = std::max(nlev, lev[s] + 1);
+    for (int c = h->super[s]; c < h->super[s + 1]; ++c) col2sup[c] = s;
+  }
+  std::vector<SelinvNode> nodes;
+  h->sel_lvl.assign(nlev + 1, 0);
+  int64_t arena = 0;
+  for (int l = nlev - 1, k = 0; l >= 0; --l, ++k) {
+    int64_t off = 0;
+    for (int s = 0; s < ns; ++s) {
+      if (lev[s] != l) continue;
+      SelinvNode nd;
+      nd.vbase = h->px[s];
+      nd.w = h->super[s + 1] - h->super[s];
+      nd.nrow = (int)(h->pi[s + 1] - h->pi[s]);
+      nd.c0 = h->super[s];
+      nd.rows_off = (int)h->pi[s];
+      nd.scratch = off;
+      const int64_t m = nd.nrow - nd.w;
+      off += (int64_t)nd.w * nd.w + m * nd.w + m * m;
+      nodes.push_back(nd);
+    }
+    h->sel_lvl[k + 1] = (int)nodes.size();
+    arena = std::max(arena, off);
+  }
+  int rc = SWC_OK;
+  if (!rc) rc = upload(&h->dNodes, nodes.data(), nodes.size(), st);
+  if (!rc) rc = upload(&h->dCol2sup, col2sup.data(), col2sup.size(), st);
+  if (!rc) rc = upload(&h->dSpx, h->px.data(), h->px.size(), st);
+  if (!rc) rc = upload(&h->dSsuper, h->super.data(), h->super.size(), st);
+  if (!rc) rc = upload(&h->dSpi, h->pi.data(), h->pi.size(), st);
+  if (!rc) rc = upload(&h->dArena, (const double*)nullptr, (size_t)std::max<int64_t>(arena, 1), st);
+  if (!rc) rc = upload(&h->dZ, (const double*)nullptr, (size_t)h->plan.xsize, st);
+  if (rc) return rc;
+  SWC_HIP(hipStreamSynchronize(st));
+  return SWC_OK;
+}
+
+int selinv_device(swc_handle* h) {
+  if (h->zvalid) return SWC_OK;
+  if (int rc = selinv_setup(h)) return rc;
+  hipStream_t st = h->stream;
+  const int nl = (int)h->sel_lvl.size() - 1;
+  for (int k = 0; k < nl; ++k) {
+    const int n0 = h->sel_lvl[k], nn = h->sel_lvl[k + 1] - n0;
+    if (nn > 0)
+      hipLaunchKernelGGL(k_selinv, dim3(nn), dim3(kThreads), 0, st, (const double*)h->dL, h->dZ,
+                         (const int*)h->dLs, (const int*)h->dCol2sup, (const int64_t*)h->dSpx,
+                         (const int*)h->dSsuper, (const int64_t*)h->dSpi,
+                         (const SelinvNode*)h->dNodes, n0, h->dArena);
+  }
+  SWC_HIP(hipGetLastError());
+  SWC_HIP(hipStreamSynchronize(st));
+  h->zvalid = true;
+  return SWC_OK;
+}
+
 int check_handle(const swc_handle* h) {
   if (!h) return swc_set_error(SWC_EINVAL, "null handle");
   return SWC_OK;
@@ -1030,6 +1362,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (blockSet[0] != 0 || blockSet[supNo] != n)
     return swc_set_error(SWC_EINVAL, "swc_upload: blockSet must span 0..n");
   h->factored = false;
+  h->drop_graphs();
   h->n = n;
   h->nsuper = supNo;
   h->super.assign(blockSet, blockSet + supNo + 1);
@@ -1042,6 +1375,8 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   }
   const int64_t ssize = h->pi[supNo];
   h->Ls.assign(Ls, Ls + ssize);
+  h->pruneptr.assign(prunePtr, prunePtr + supNo + 1);
+  h->pruneset.assign(pruneSet, pruneSet + prunePtr[supNo]);
   try {
     swc::build_plan(n, supNo, h->super.data(), h->pi.data(), h->px.data(), h->Ls.data(),
                     prunePtr, pruneSet, h->plan);
@@ -1083,6 +1418,20 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
       }
   }
   h->nnz_full = frp[n];
+  // factor-order coordinates of every input value, and the diagonal positions
+  std::vector<int> qi(h->nvals, 0), qj(h->nvals, 0);
+  for (int j = 0; j < n; ++j)
+    for (int q = A2p[j]; q < A2p[j + 1]; ++q) {
+      const int64_t dst = a2src ? a2src[q] : q;
+      qi[dst] = A2i[q];
+      qj[dst] = j;
+    }
+  std::vector<int64_t> dpos(n);
+  for (int s = 0; s < supNo; ++s) {
+    const int64_t R = h->pi[s + 1] - h->pi[s];
+    for (int c = blockSet[s]; c < blockSet[s + 1]; ++c)
+      dpos[c] = h->px[s] + (int64_t)(c - blockSet[s]) * R + (c - blockSet[s]);
+  }
   const Plan& P = h->plan;
   hipStream_t st = h->stream;
   int rc = SWC_OK;
@@ -1097,6 +1446,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dSblkPtr, P.sblk_ptr.data(), P.sblk_ptr.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
   if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
+  if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
   if (!rc) rc = upload(&h->dAmap, amap.data(), amap.size(), st);
   if (!rc) rc = upload(&h->dVals, (const double*)nullptr, (size_t)h->nvals, st);
   if (!rc) rc = upload(&h->dRp, frp.data(), frp.size(), st);
@@ -1104,6 +1454,17 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dVmap, fvm.data(), fvm.size(), st);
   if (!rc) rc = upload(&h->dAf, (const double*)nullptr, fci.size(), st);
   if (!rc) rc = upload(&h->dScal, (const double*)nullptr, 64, st);
+  if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);
+  if (!rc) rc = upload(&h->dQi, qi.data(), qi.size(), st);
+  if (!rc) rc = upload(&h->dQj, qj.data(), qj.size(), st);
+  if (!rc) rc = upload(&h->dDiagPos, dpos.data(), dpos.size(), st);
+  h->zvalid = false;
+  hipFree(h->dZ);
+  hipFree(h->dArena);
+  hipFree(h->dNodes);
+  h->dZ = nullptr;
+  h->dArena = nullptr;
+  h->dNodes = nullptr;
   if (rc) return rc;
   SWC_HIP(hipStreamSynchronize(st));
   return SWC_OK;
@@ -1244,6 +1605,70 @@ int swc_set_precision(swc_handle* h, int mode) {
   if (mode < 0 || mode > 2) return swc_set_error(SWC_EINVAL, "swc_set_precision: mode must be 0, 1 or 2");
   h->precision = mode;
   h->fp32_mask = mode == 0 ? 0 : (mode == 1 ? 1 : 3);
+  return SWC_OK;
+}
+
+int swc_selected_inverse(swc_handle* h, double* Z, int on_device) {
+  if (int rc = check_handle(h)) return rc;
+  if (!h->factored || h->last_info) return swc_set_error(SWC_ESTATE, "swc_selected_inverse: no valid factor");
+  if (int rc = selinv_device(h)) return rc;
+  if (Z) {
+    SWC_HIP(hipMemcpyAsync(Z, h->dZ, sizeof(double) * h->plan.xsize,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+    SWC_HIP(hipStreamSynchronize(h->stream));
+  }
+  return SWC_OK;
+}
+
+int swc_logdet(swc_handle* h, double* logdet) {
+  if (int rc = check_handle(h)) return rc;
+  if (!logdet) return swc_set_error(SWC_EINVAL, "swc_logdet: null argument");
+  if (!h->factored || h->last_info) return swc_set_error(SWC_ESTATE, "swc_logdet: no valid factor");
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL(k_logdet_partial, dim3(kRedBlocks), dim3(kThreads), 0, st, (const double*)h->dL,
+                     (const int64_t*)h->dDiagPos, h->n, h->dNormPart);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, (const double*)h->dNormPart, 1,
+                     h->dScal + 1);
+  double v = 0;
+  SWC_HIP(hipMemcpyAsync(&v, h->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  SWC_HIP(hipStreamSynchronize(st));
+  *logdet = 2.0 * v;
+  return SWC_OK;
+}
+
+int swc_gradient(swc_handle* h, const double* x, int x_on_device, double alpha, double beta,
+                 double* grad, int grad_on_device) {
+  if (int rc = check_handle(h)) return rc;
+  if (!grad || (beta != 0.0 && !x)) return swc_set_error(SWC_EINVAL, "swc_gradient: null argument");
+  if (!h->factored || h->last_info) return swc_set_error(SWC_ESTATE, "swc_gradient: no valid factor");
+  if (alpha != 0.0)
+    if (int rc = selinv_device(h)) return rc;
+  if (int rc = ensure_work(h, 1)) return rc;
+  hipStream_t st = h->stream;
+  double* xt = h->dRef;
+  if (beta != 0.0)
+    if (int rc = to_factor(h, x, x_on_device, h->n, xt, 1)) return rc;
+  double* g = grad_on_device ? grad : h->dRef + h->n;  // dRef holds >= 5 n doubles
+  double* gbuf = nullptr;
+  if (!grad_on_device) {
+    SWC_HIP(hipMallocAsync((void**)&gbuf, sizeof(double) * std::max<int64_t>(h->nvals, 1), st));
+    g = gbuf;
+  }
+  hipLaunchKernelGGL(k_grad_values, dim3(grid_for(h->nvals)), dim3(kThreads), 0, st,
+                     (const double*)h->dZ, (const int64_t*)h->dAmap, (const int*)h->dQi,
+                     (const int*)h->dQj, (const double*)xt, alpha, beta, h->nvals, g);
+  SWC_HIP(hipGetLastError());
+  if (!grad_on_device) {
+    SWC_HIP(hipMemcpyAsync(grad, gbuf, sizeof(double) * h->nvals, hipMemcpyDeviceToHost, st));
+    SWC_HIP(hipFreeAsync(gbuf, st));
+  }
+  SWC_HIP(hipStreamSynchronize(st));
+  return SWC_OK;
+}
+
+int swc_set_graphs(swc_handle* h, int on) {
+  if (int rc = check_handle(h)) return rc;
+  h->use_graphs = on != 0;
   return SWC_OK;
 }
 

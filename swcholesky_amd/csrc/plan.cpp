@@ -124,7 +124,8 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     pt.t.r0 = r0;
     pt.t.nr = std::min(TM, pn.nrow - r0);
     pt.t.kind = kind;
-    pt.t.pad0 = pt.t.pad1 = 0;
+    pt.t.slot = -1;
+    pt.t.pad = 0;
     pt.t.task_begin = pt.t.task_end = 0;
     pt.level = level;
     pt.trsm = trsm;
@@ -254,10 +255,37 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     (void)trow;
     pt.t.task_end = (int)P.tasks.size();
     P.lvl_tiles[pt.level + 1]++;
-    if (pt.trsm) trsm_by_level[pt.level].push_back((int)P.tiles.size());
+    if (pt.trsm || pt.t.kind == 0) trsm_by_level[pt.level].push_back((int)P.tiles.size());
     P.tiles.push_back(pt.t);
   }
   for (int l = 0; l < P.nlevel; ++l) P.lvl_tiles[l + 1] += P.lvl_tiles[l];
+  // scratch slots for the diagonal blocks each level factors: the update kernel
+  // copies the diagonal tile's block there, every factor-kernel tile reads it
+  for (int l = 0; l < P.nlevel; ++l) {
+    std::vector<int> slot_of_panel;
+    int64_t off = 0;
+    std::vector<std::pair<int, int>> diag_slots;  // panel, slot
+    for (int ti : trsm_by_level[l]) {
+      Tile& tl = P.tiles[ti];
+      if (tl.kind == 0) {
+        const int nc = P.panels[tl.panel].ncol;
+        tl.slot = (int)off;
+        diag_slots.push_back({tl.panel, (int)off});
+        off += (int64_t)nc * nc;
+      }
+    }
+    std::sort(diag_slots.begin(), diag_slots.end());
+    for (int ti : trsm_by_level[l]) {
+      Tile& tl = P.tiles[ti];
+      if (tl.kind == 0) continue;
+      auto it = std::lower_bound(diag_slots.begin(), diag_slots.end(), std::make_pair(tl.panel, -1));
+      if (it == diag_slots.end() || it->first != tl.panel)
+        throw std::logic_error("panel factored without its diagonal tile");
+      tl.slot = it->second;
+    }
+    if (off > 0x7fffffff) throw std::invalid_argument("diagonal scratch too large");
+    P.diag_scratch = std::max(P.diag_scratch, off);
+  }
   for (int l = 0; l < P.nlevel; ++l) {
     P.trsm_tiles.insert(P.trsm_tiles.end(), trsm_by_level[l].begin(), trsm_by_level[l].end());
     P.lvl_trsm[l + 1] = (int)P.trsm_tiles.size();
@@ -265,9 +293,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   for (const Panel& pn : P.panels) {
     const double w = pn.ncol, m = pn.nrow - pn.ncol;
     P.factor_flops += w * w * w / 3.0 + m * w * w;
-    const double m0 = std::min(pn.nrow, TM) - w;  // rows solved inside the diagonal tile
-    P.tile_kernel_flops += w * w * w / 3.0 + m0 * w * w;
-    P.trsm_kernel_flops += (m - m0) * w * w;
+    P.trsm_kernel_flops += w * w * w / 3.0 + m * w * w;  // POTRF + TRSM of the panel
   }
 }
 
@@ -316,6 +342,7 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[7] = (int64_t)P.sblocks.size();
   out[8] = P.vsize;
   out[9] = P.max_lvl_sblk;
+  out[10] = P.diag_scratch;
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;

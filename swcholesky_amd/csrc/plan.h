@@ -39,8 +39,10 @@ struct Tile {
   int nr;          // rows in the tile (<= TM)
   int task_begin;  // [task_begin, task_end) into the task array
   int task_end;
-  int kind;        // 0: diagonal tile (POTRF + own TRSM), 1: off-diagonal
-  int pad0, pad1;
+  int kind;        // 0: the panel's diagonal tile, 1: below it
+  int slot;        // factor level: offset of the panel's diagonal block copy in the
+                   // per-level scratch (updated, not yet factored), -1 otherwise
+  int pad;
 };
 
 // One source contribution to one tile: rows [ra,rb) x columns [ca,cb) of the
@@ -79,7 +81,7 @@ struct Plan {
   std::vector<Panel> panels;
   std::vector<Tile> tiles;          // grouped by level (one update launch per level)
   std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
-  std::vector<int> trsm_tiles;      // tiles solved against their panel's diagonal block
+  std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)
   std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles
   std::vector<Task> tasks;
   std::vector<int> panel_level;     // per panel
@@ -91,10 +93,11 @@ struct Plan {
   std::vector<int> sblk_ptr;        // npanel+1 into sblocks (panels in panel_order)
   std::vector<int> lvl_sblk;        // nlevel+1 into sblocks
   int max_lvl_sblk = 0;             // most row blocks in one level
+  int64_t diag_scratch = 0;         // doubles of diagonal-block copies one level needs
   int64_t vsize = 0;                // sum over supernodes of (rows - width)
   double update_flops = 0, factor_flops = 0;
-  // algorithmic flops of each kernel class: lower-triangle update products and
-  // the diagonal-tile POTRF + in-tile TRSM (tile kernel), off-diagonal TRSM
+  // algorithmic flops of each kernel class: lower-triangle update products (tile
+  // kernel); POTRF + TRSM of every panel, counted once (panel factor kernel)
   double tile_kernel_flops = 0, trsm_kernel_flops = 0;
   int nlevel = 0;
 };

@@ -18,6 +18,7 @@ def factor(plan, Ls, xsize, amap, vals):
     panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
     info = 0
     for lvl in range(plan["nlevel"]):
+        scratch = {}
         t0, t1 = plan["lvl_tiles"][lvl], plan["lvl_tiles"][lvl + 1]
         s0, s1 = plan["lvl_trsm"][lvl], plan["lvl_trsm"][lvl + 1]
         for ti in range(t0, t1):
@@ -53,31 +54,39 @@ def factor(plan, Ls, xsize, amap, vals):
                 mask = trow[pr][:, None] >= (int(pn["c0"]) + pc)[None, :]
                 sub = acc[np.ix_(pr, pc)]
                 acc[np.ix_(pr, pc)] = np.where(mask, sub - C, sub)
-            if tl["kind"] == 0:
-                for j in range(nc):
-                    d = acc[j, j]
-                    if not d > 0:
-                        info = info or int(pn["c0"]) + j + 1
-                        d = 1.0
-                        acc[j, j] = 1.0
-                    else:
-                        acc[j, j] = np.sqrt(d)
-                    acc[j + 1:, j] /= acc[j, j]
-                    for c in range(j + 1, nc):
-                        acc[c:, c] -= acc[c:, j] * acc[c, j]
-                keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :]
-                L[idx[keep]] = acc[keep]
-            else:
-                L[idx] = acc
+            keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["kind"] == 0 else \
+                np.ones((nr, nc), bool)
+            L[idx[keep]] = acc[keep]
+            if tl["slot"] >= 0 and tl["kind"] == 0:
+                _chk(int(tl["slot"]), int(tl["slot"]) + nc * nc, plan["diag_scratch"], "diag slot")
+                scratch[int(tl["slot"])] = np.tril(acc[:nc, :nc]).copy()
+        # panel factor kernel: every listed tile recomputes L11 of its panel
         for ti in plan["trsm_tiles"][s0:s1]:
             tl = tiles[ti]
             pn = panels[tl["panel"]]
             nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
-            base = int(pn["vbase"]) + int(tl["r0"])
-            idx = base + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]
             didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
-            l11 = np.tril(L[didx])
-            L[idx] = np.linalg.solve(l11, L[idx].T).T
+            a11 = scratch[int(tl["slot"])].copy()  # the launch-invariant copy
+            l11 = np.zeros_like(a11)
+            for j in range(nc):  # right-looking, as the kernel does
+                d = a11[j, j]
+                if not d > 0:
+                    if tl["kind"] == 0:
+                        info = info or int(pn["c0"]) + j + 1
+                    d = 1.0
+                l11[j, j] = np.sqrt(d) if a11[j, j] > 0 else 1.0
+                l11[j + 1:, j] = a11[j + 1:, j] / l11[j, j]
+                for c in range(j + 1, nc):
+                    a11[c:, c] -= l11[c:, j] * l11[c, j]
+            first = nc if tl["kind"] == 0 else 0
+            if tl["kind"] == 0:
+                keep = np.tril(np.ones((nc, nc), bool))
+                L[didx[keep]] = l11[keep]
+            if nr > first:
+                base = int(pn["vbase"]) + int(tl["r0"])
+                idx = base + np.arange(nc)[None, :] * ld + np.arange(first, nr)[:, None]
+                _chk(int(idx.min()), int(idx.max()) + 1, xsize, "factor tile")
+                L[idx] = np.linalg.solve(l11, L[idx].T).T
     return L, info
 
 

@@ -89,3 +89,34 @@ def test_refactor_same_structure_new_values():
     assert f.factor(v2) == 0
     Lo2, _ = O.factor(b, b.a2_values(v2))
     assert relF(f.L_values(), Lo2) < 1e-12
+
+
+def test_graph_replays_match_eager_launches():
+    """Captured factor/solve sequences replayed several times, with eager work
+    and synchronizations in between, give bitwise the eager results."""
+    from swcholesky_amd import problems as P
+    prob = P.laplacian2d_const(100)
+    b, a2x, Lo, _ = setup(prob)
+    f = S.Factor(b)
+    f.set_graphs(False)
+    assert f.factor(prob.values) == 0
+    L_eager = f.L_values()
+    rhs = np.random.default_rng(5).uniform(-1, 1, (prob.n, 3))
+    x_eager = f.solve(rhs)
+    f.set_graphs(True)
+    for rep in range(4):
+        vals = prob.values * (1.0 + 0.5 * rep)
+        assert f.factor(vals) == 0
+        f.set_graphs(False)  # eager work between replays
+        Lo_rep, _ = O.factor(b, b.a2_values(vals))
+        f.set_graphs(True)
+        assert relF(f.L_values(), Lo_rep) < 1e-12
+    assert f.factor(prob.values) == 0
+    assert np.array_equal(f.L_values(), L_eager)
+    for rep in range(3):
+        assert np.array_equal(f.solve(rhs), x_eager)
+        _ = f.solve(rhs[:, :1])  # a different shape in between
+    for mode in (1, 0):  # precision switches reuse their own captured sequence
+        f.set_precision(mode)
+        assert f.factor(prob.values) == 0
+    assert np.array_equal(f.L_values(), L_eager)
