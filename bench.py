@@ -62,6 +62,16 @@ def main():
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--cpu-side", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="factor steps only (for counter runs): no refinement paths, no "
+                         "per-launch event pass, no CPU baseline")
+    ap.add_argument("--workload", default="lap3d", choices=["lap3d", "vec3d", "batch", "gmrf"],
+                    help="lap3d: the headline 64^3 operator (default); vec3d: 48^3 x 3-dof "
+                         "operator, mixed precision with 1/8/32 RHS; batch: 4096 uneven 2D "
+                         "instances in one launch sequence; gmrf: 256^2 GMRF gradient")
+    ap.add_argument("--copy-bytes", type=float, default=0,
+                    help="also run one device-to-device copy of this many bytes (a plain "
+                         "copy kernel of known size, to scale HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -71,6 +81,9 @@ def main():
     import torch
     import swcholesky_amd as S
     from swcholesky_amd import problems as P
+
+    if args.workload != "lap3d":
+        return OTHER[args.workload](args)
 
     log(f"building operator {args.side}^3")
     prob = P.laplacian3d_var(args.side)
@@ -99,6 +112,19 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     log(f"steps: wall {ms_per_step:.2f} ms/step, device {np.median(ev_ms):.2f} ms")
     gflops = b.flops / (ms_per_step * 1e-3) / 1e9
+
+    if args.copy_bytes:
+        nbytes = int(args.copy_bytes) // 8 * 8
+        src_t = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        torch.cuda.synchronize()
+        dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        log(f"copy kernel: {nbytes} bytes read, {nbytes} bytes written")
+    if args.quick:
+        print(json.dumps({"metric": "factor GFLOP/s (quick counter run)", "value": gflops,
+                          "unit": "GFLOP/s", "ms_per_step": ms_per_step}), flush=True)
+        return
 
     # time to 1e-12 backward error: factor + solve + FP64 refinement, with the
     # values and the right-hand side resident on the device
@@ -169,13 +195,133 @@ def main():
         "time_to_1e-12_ms": {k: v["ms"] for k, v in paths.items()},
         "mixed_speedup_vs_fp64": paths["fp64"]["ms"] / min(paths["mixed"]["ms"], paths["mixed_all"]["ms"]),
         "paths": paths,
-        "roofline": {"bound": "mfma", "kernel": "k_tile_update_factor", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "k_tile_update", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
+
+def _time_refine(f, drhs, tol=1e-12):
+    import torch
+    x = drhs.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, eta, iters = f.refine(x, tol=tol, maxiter=40)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3, eta, iters, x
+
+
+def bench_vec3d(args):
+    """48^3 x 3-dof operator (n = 331,776): FP64 and mixed factor, then refinement
+    to 1e-12 for 1, 8 and 32 right-hand sides ~ U(-1, 1)."""
+    import torch
+    import swcholesky_amd as S
+    from swcholesky_amd import problems as P
+    side = 48 if args.side == 64 else args.side
+    prob = P.vector3d_var(side)
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    log(f"vec3d {side}^3x3: n={prob.n} nsuper={b.nsuper} xsize={b.xsize} F={b.flops:.4e}")
+    f = S.Factor(b)
+    dvals = torch.from_numpy(prob.values).cuda()
+    rng = np.random.default_rng(7)
+    res = {}
+    for name, mode in (("fp64", 0), ("mixed", 1), ("mixed_all", 2)):
+        f.set_precision(mode)
+        f.factor(dvals)  # warm-up (captures the launch sequence)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = f.factor(dvals)
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t0) * 1e3
+        rec = {"factor_ms": tf, "factor_gflops": b.flops / tf / 1e6, "info": info}
+        for nrhs in (1, 8, 32):
+            B = torch.from_numpy(np.ascontiguousarray(rng.uniform(-1, 1, (nrhs, prob.n)))).cuda()
+            _time_refine(f, B)  # warm-up of this shape
+            ms, eta, iters, _ = _time_refine(f, B)
+            rec[f"refine_{nrhs}rhs"] = {"ms": ms, "eta_max": float(np.max(eta)), "iters": iters}
+        res[name] = rec
+        log(f"{name}: {rec}")
+    out = {"metric": "factor GFLOP/s & time-to-1e-12 backward error, mixed vs FP64, 1 MI355X",
+           "workload": f"vec3d_var_{side}: 3-dof couplings w(I+uu^T/2), n={prob.n}",
+           "value": res["fp64"]["factor_gflops"], "unit": "GFLOP/s", "dtype": "f64",
+           "flops_F": b.flops, "paths": res}
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """4096 independent 2D 5-point variable-coefficient Laplacians, side ~ U{16..128},
+    weights log-uniform [1e-1, 1e1] (seed 3), shift 1e-4: factored together
+    (one launch per elimination-tree level), then one RHS each refined to 1e-12."""
+    import torch
+    from swcholesky_amd import problems as P
+    from swcholesky_amd.batch import Batch
+    count = 4096 if args.side == 64 else args.side
+    rng = np.random.default_rng(3)
+    sides = rng.integers(16, 129, size=count)
+    t0 = time.perf_counter()
+    probs = [P.laplacian2d_var(int(sd), seed=None, rng=rng, wlo=1e-1, whi=1e1, shift=1e-4)
+             for sd in sides]
+    bt = Batch(probs)
+    log(f"batch of {count}: n={bt.n} nsuper={bt.bcsc.nsuper} F={bt.bcsc.flops:.4e} "
+        f"host prep {time.perf_counter() - t0:.1f} s")
+    f = bt.factor
+    dvals = torch.from_numpy(bt.values(probs)).cuda()
+    rhs = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, bt.n)).cuda()
+    res = {}
+    for name, mode in (("fp64", 0), ("mixed", 1)):
+        f.set_precision(mode)
+        f.factor(dvals)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = f.factor(dvals)
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t0) * 1e3
+        _time_refine(f, rhs)
+        ms, eta, iters, _ = _time_refine(f, rhs)
+        res[name] = {"factor_ms": tf, "factor_gflops": bt.bcsc.flops / tf / 1e6, "info": info,
+                     "refine_ms": ms, "eta": float(eta[0]), "iters": iters,
+                     "time_to_1e-12_ms": tf + ms}
+        log(f"{name}: {res[name]}")
+    print(json.dumps({"metric": "factor GFLOP/s & time-to-1e-12 backward error, mixed vs FP64",
+                      "workload": f"{count} uneven 2D instances", "n": bt.n,
+                      "value": res["fp64"]["factor_gflops"], "unit": "GFLOP/s",
+                      "flops_F": bt.bcsc.flops, "paths": res}), flush=True)
+
+
+def bench_gmrf(args):
+    """GMRF Q = tau(kappa^2 I + L_grid) on 256^2: factor, logdet + b^T Q^-1 b and its
+    gradient w.r.t. Q's stored values (supernodal selected inverse)."""
+    import torch
+    import swcholesky_amd as S
+    from swcholesky_amd import problems as P
+    from swcholesky_amd.autograd import logdet_quad
+    side = 256 if args.side == 64 else args.side
+    prob = P.gmrf2d(side)
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    f = S.Factor(b)
+    vals = torch.tensor(prob.values, device="cuda", requires_grad=True)
+    rhs = torch.tensor(np.random.default_rng(4).standard_normal(prob.n), device="cuda")
+    for rep in range(2):
+        vals.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = logdet_quad(vals, rhs, f)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    res = {"forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "loss": float(out),
+           "n": prob.n, "flops_F": b.flops}
+    log(f"gmrf: {res}")
+    print(json.dumps({"metric": "gradient of logdet(Q) + b^T Q^-1 b (256^2 GMRF)",
+                      "workload": f"gmrf2d_{side}", "value": res["backward_ms"], "unit": "ms",
+                      "higher_is_better": False, "details": res}), flush=True)
+
+
+OTHER = {"vec3d": bench_vec3d, "batch": bench_batch, "gmrf": bench_gmrf}
 
 if __name__ == "__main__":
     main()

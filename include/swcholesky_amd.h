@@ -173,6 +173,8 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
 void swc_plan_free(swc_plan* p);
 int swc_plan_sizes(const swc_plan* p, int64_t* sizes, double* flops);
 int swc_plan_export(const swc_plan* p, int which, void* dst);
+/* Byte sizes of the exported structs {Panel, Tile, Task, SolveTask, SolveBlock}. */
+int swc_plan_struct_sizes(int64_t* out);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,7 @@ SIGNATURES = {
     "swc_plan_free": (None, [c_vp]),
     "swc_plan_sizes": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_plan_export": (c_int, [c_vp, c_int, c_vp]),
+    "swc_plan_struct_sizes": (c_int, [c_vp]),
 }
 
 _lib = None

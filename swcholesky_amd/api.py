@@ -245,7 +245,9 @@ TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("slot", np.int32), ("pad", np.int32)])
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
-                    ("ca", np.int32), ("cb", np.int32), ("kind", np.int32)])
+                    ("ca", np.int32), ("cb", np.int32), ("kind", np.int32),
+                    ("ti0", np.int32), ("tj0", np.int32), ("pad0", np.int32),
+                    ("pad1", np.int32)])
 STASK_DT = np.dtype([("voff", np.int64), ("rows_off", np.int32), ("ca", np.int32),
                      ("cb", np.int32), ("wd", np.int32)])
 SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("pad", np.int32)])
@@ -254,6 +256,12 @@ SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("p
 def build_plan(bcsc: BCSC):
     """The device work plan for a structure, as numpy record arrays (host only)."""
     lib = _lib.lib()
+    sizes = np.zeros(5, dtype=np.int64)
+    check(lib.swc_plan_struct_sizes(ptr(sizes)))
+    want = [PANEL_DT.itemsize, TILE_DT.itemsize, TASK_DT.itemsize, STASK_DT.itemsize,
+            SBLK_DT.itemsize]
+    if list(sizes) != want:
+        raise RuntimeError(f"plan struct layout mismatch: library {list(sizes)}, python {want}")
     h = ctypes.c_void_p()
     super_ = np.ascontiguousarray(bcsc.super)
     p = np.ascontiguousarray(bcsc.p.astype(np.uint64))

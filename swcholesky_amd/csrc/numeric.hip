@@ -125,7 +125,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // The same product with operands rounded to FP32 and FP32 accumulation
 // (v_mfma_f32_16x16x4f32, D[4*(l/16)+r][l%16] in component r); the caller
 // subtracts the FP32 result from its FP64 accumulator.
-__device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int k, int ra, int M,
+__device__ inline void task_mfma_f32(const float* __restrict__ src, int ld, int k, int ra, int M,
                                      int ca, int N, int wave, int lane, f4 (&c)[2][2]) {
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 #pragma unroll
@@ -138,8 +138,8 @@ __device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int
   const int q0 = wc + lr, q1 = wc + 16 + lr;
   const bool a0ok = r0 < M, a1ok = r1 < M, b0ok = q0 < N, b1ok = q1 < N;
   const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;
-  const double* pa = src + ra;
-  const double* pb = src + ca;
+  const float* pa = src + ra;
+  const float* pb = src + ca;
   for (int kk = 0; kk < k; kk += 16) {
     float a0[4], a1[4], b0[4], b1[4];
 #pragma unroll
@@ -147,10 +147,10 @@ __device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int
       const int kc = kk + 4 * u + lk;
       const bool kok = kc < k;
       const int64_t off = (int64_t)kc * ld;
-      a0[u] = (a0ok && kok) ? (float)pa[off + r0] : 0.f;
-      a1[u] = (a1ok && kok) ? (float)pa[off + r1] : 0.f;
-      b0[u] = (b0ok && kok) ? (float)pb[off + q0] : 0.f;
-      b1[u] = (b1ok && kok) ? (float)pb[off + q1] : 0.f;
+      a0[u] = (a0ok && kok) ? pa[off + r0] : 0.f;
+      a1[u] = (a1ok && kok) ? pa[off + r1] : 0.f;
+      b0[u] = (b0ok && kok) ? pb[off + q0] : 0.f;
+      b1[u] = (b1ok && kok) ? pb[off + q1] : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -164,15 +164,109 @@ __device__ inline void task_mfma_f32(const double* __restrict__ src, int ld, int
   }
 }
 
-// Tile kernel: accumulate all source updates of one tile in LDS (in task order)
-// and write the tile back.  The factorization of panels completed by this
-// level follows in k_panel_factor.
+// Contiguous task on FP64 MFMA straight into the tile accumulator: acc -= A B^T
+// where tile row tr takes source row ra + tr - ti0 and tile column tc takes
+// source row ca + tc - tj0 (rows/columns outside the task block read as zero).
+__device__ inline void dense_mfma_f64(const double* __restrict__ src, int ld, int k, int ra, int M,
+                                      int ca, int N, int ti0, int tj0, int wave, int lane,
+                                      d4 (&acc)[2][2]) {
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  if (wr + 32 <= ti0 || wr >= ti0 + M || wc + 32 <= tj0 || wc >= tj0 + N) return;  // uniform
+  const int lr = lane & 15, lk = lane >> 4;
+  const int sa0 = wr + lr - ti0, sa1 = sa0 + 16;      // source row offsets of the A operand
+  const int sb0 = wc + lr - tj0, sb1 = sb0 + 16;
+  const bool a0ok = sa0 >= 0 && sa0 < M, a1ok = sa1 >= 0 && sa1 < M;
+  const bool b0ok = sb0 >= 0 && sb0 < N, b1ok = sb1 >= 0 && sb1 < N;
+  const bool ba0 = wr + 16 > ti0 && wr < ti0 + M, ba1 = wr + 32 > ti0 && wr + 16 < ti0 + M;
+  const bool bb0 = wc + 16 > tj0 && wc < tj0 + N, bb1 = wc + 32 > tj0 && wc + 16 < tj0 + N;
+  const double* pa = src + ra;
+  const double* pb = src + ca;
+  for (int kk = 0; kk < k; kk += 16) {
+    double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kc = kk + 4 * u + lk;
+      const bool kok = kc < k;
+      const int64_t off = (int64_t)kc * ld;
+      a0[u] = (a0ok && kok) ? -pa[off + sa0] : 0.0;
+      a1[u] = (a1ok && kok) ? -pa[off + sa1] : 0.0;
+      b0[u] = (b0ok && kok) ? pb[off + sb0] : 0.0;
+      b1[u] = (b1ok && kok) ? pb[off + sb1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (ba0 && bb0) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0][0], 0, 0, 0);
+      if (ba0 && bb1) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[0][1], 0, 0, 0);
+      if (ba1 && bb0) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[1][0], 0, 0, 0);
+      if (ba1 && bb1) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[1][1], 0, 0, 0);
+    }
+  }
+}
+
+// The same on FP32 MFMA with FP32 accumulation, subtracted into the FP64
+// accumulator.  The A operand rows are fed in the order pi(r) = r/4 + 4(r%4),
+// which makes the FP32 output land in the FP64 variant's register layout.
+__device__ inline void dense_mfma_f32(const float* __restrict__ src, int ld, int k, int ra, int M,
+                                      int ca, int N, int ti0, int tj0, int wave, int lane,
+                                      d4 (&acc)[2][2]) {
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  if (wr + 32 <= ti0 || wr >= ti0 + M || wc + 32 <= tj0 || wc >= tj0 + N) return;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int pr_ = (lr >> 2) + 4 * (lr & 3);
+  const int sa0 = wr + pr_ - ti0, sa1 = sa0 + 16;
+  const int sb0 = wc + lr - tj0, sb1 = sb0 + 16;
+  const bool a0ok = sa0 >= 0 && sa0 < M, a1ok = sa1 >= 0 && sa1 < M;
+  const bool b0ok = sb0 >= 0 && sb0 < N, b1ok = sb1 >= 0 && sb1 < N;
+  const bool ba0 = wr + 16 > ti0 && wr < ti0 + M, ba1 = wr + 32 > ti0 && wr + 16 < ti0 + M;
+  const bool bb0 = wc + 16 > tj0 && wc < tj0 + N, bb1 = wc + 32 > tj0 && wc + 16 < tj0 + N;
+  const float* pa = src + ra;
+  const float* pb = src + ca;
+  f4 c[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int kk = 0; kk < k; kk += 16) {
+    float a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kc = kk + 4 * u + lk;
+      const bool kok = kc < k;
+      const int64_t off = (int64_t)kc * ld;
+      a0[u] = (a0ok && kok) ? pa[off + sa0] : 0.f;
+      a1[u] = (a1ok && kok) ? pa[off + sa1] : 0.f;
+      b0[u] = (b0ok && kok) ? pb[off + sb0] : 0.f;
+      b1[u] = (b1ok && kok) ? pb[off + sb1] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (ba0 && bb0) c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], c[0][0], 0, 0, 0);
+      if (ba0 && bb1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], c[0][1], 0, 0, 0);
+      if (ba1 && bb0) c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], c[1][0], 0, 0, 0);
+      if (ba1 && bb1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], c[1][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[bi][bj][r] -= (double)c[bi][bj][r];
+}
+
+// Tile kernel: applies all source updates of one tile, in task order, to an
+// accumulator held in MFMA registers (lane l, component r of block (bi, bj) of
+// wave w is tile element (wr + 16 bi + l/16 + 4 r, wc + 16 bj + l%16)), then
+// writes the tile back.  Contiguous tasks multiply straight into the registers;
+// a scattered task computes its block, scatters it through an LDS staging tile
+// at its mapped rows and columns, and every lane pulls back its own elements.
+// The factorization of panels completed by this level follows in k_panel_factor.
 __global__ void __launch_bounds__(kThreads)
 k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ panels, const Tile* __restrict__ tiles,
               const Task* __restrict__ tasks, int tile0, int fp32_mask,
-              double* __restrict__ dscratch) {
-  __shared__ double acc[swc::PW * kLdsLd];
+              double* __restrict__ dscratch, const float* __restrict__ Lf) {
+  __shared__ double stage[swc::PW * kLdsLd];
   __shared__ int trow[swc::TM];
   __shared__ int pr[swc::TM];
   __shared__ int pc[swc::TM];
@@ -180,36 +274,50 @@ k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
   const Panel pn = panels[tl.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = tl.nr, nc = pn.ncol;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   double* gtile = L + pn.vbase + tl.r0;
-  for (int e = tid; e < nr; e += kThreads) trow[e] = Ls[pn.rows_off + tl.r0 + e];
-  for (int e = tid; e < nr * nc; e += kThreads) {
-    const int i = e % nr, j = e / nr;
-    acc[j * kLdsLd + i] = gtile[(int64_t)j * pn.ld + i];
-  }
-  __syncthreads();
+  d4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
+        acc[bi][bj][r] = (i < nr && j < nc) ? gtile[(int64_t)j * pn.ld + i] : 0.0;
+      }
+  bool have_rows = false;
   for (int t = tl.task_begin; t < tl.task_end; ++t) {
     const Task tk = tasks[t];
     const int M = tk.rb - tk.ra, N = tk.cb - tk.ca;
+    const double* src = L + tk.src_base;
+    const bool f32 = (fp32_mask >> tk.kind) & 1;
+    if (tk.ti0 >= 0) {
+      if (f32) dense_mfma_f32(Lf + tk.src_base, tk.ld, tk.k, tk.ra, M, tk.ca, N, tk.ti0, tk.tj0, wave, lane, acc);
+      else dense_mfma_f64(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, tk.ti0, tk.tj0, wave, lane, acc);
+      continue;
+    }
+    // scattered task: map its rows and columns, stage its block, pull it back
+    if (!have_rows) {
+      for (int e = tid; e < nr; e += kThreads) trow[e] = Ls[pn.rows_off + tl.r0 + e];
+      have_rows = true;
+    }
+    for (int e = tid; e < swc::PW * kLdsLd; e += kThreads) stage[e] = 0.0;
     for (int e = tid; e < M; e += kThreads) pr[e] = find_pos(trow, nr, Ls[tk.rows_off + tk.ra + e]);
     for (int e = tid; e < N; e += kThreads) pc[e] = Ls[tk.rows_off + tk.ca + e] - pn.c0;
     __syncthreads();
-    const double* src = L + tk.src_base;
     if (tk.k < 8) {
-      // short inner dimension: one thread per output element
       for (int e = tid; e < M * N; e += kThreads) {
         const int i = e % M, j = e / M;
-        const int ti = pr[i], tj = pc[j];
-        if (trow[ti] < pn.c0 + tj) continue;  // strict upper part of the diagonal block
         const double* a = src + tk.ra + i;
         const double* b = src + tk.ca + j;
-        double s = 0.0;
-        for (int kk = 0; kk < tk.k; ++kk) s = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], s);
-        acc[tj * kLdsLd + ti] -= s;
+        double sacc = 0.0;
+        for (int kk = 0; kk < tk.k; ++kk) sacc = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], sacc);
+        stage[pc[j] * kLdsLd + pr[i]] = sacc;
       }
-    } else if ((fp32_mask >> tk.kind) & 1) {
+    } else if (f32) {
       f4 c[2][2];
-      task_mfma_f32(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
-      const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+      task_mfma_f32(Lf + tk.src_base, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -218,15 +326,11 @@ k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
           for (int r = 0; r < 4; ++r) {
             const int i = wr + 16 * bi + 4 * (lane >> 4) + r;
             const int j = wc + 16 * bj + (lane & 15);
-            if (i < M && j < N) {
-              const int ti = pr[i], tj = pc[j];
-              if (trow[ti] >= pn.c0 + tj) acc[tj * kLdsLd + ti] -= (double)c[bi][bj][r];
-            }
+            if (i < M && j < N) stage[pc[j] * kLdsLd + pr[i]] = (double)c[bi][bj][r];
           }
     } else {
       d4 c[2][2];
       task_mfma(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
-      const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -235,26 +339,41 @@ k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
           for (int r = 0; r < 4; ++r) {
             const int i = wr + 16 * bi + (lane >> 4) + 4 * r;
             const int j = wc + 16 * bj + (lane & 15);
-            if (i < M && j < N) {
-              const int ti = pr[i], tj = pc[j];
-              if (trow[ti] >= pn.c0 + tj) acc[tj * kLdsLd + ti] -= c[bi][bj][r];
-            }
+            if (i < M && j < N) stage[pc[j] * kLdsLd + pr[i]] = c[bi][bj][r];
           }
     }
     __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
+          acc[bi][bj][r] -= stage[j * kLdsLd + i];
+        }
+    __syncthreads();
   }
-  for (int e = tid; e < nr * nc; e += kThreads) {
-    const int i = e % nr, j = e / nr;
-    if (tl.kind == 0 && i < j) continue;  // keep the strict upper part untouched (zero)
-    gtile[(int64_t)j * pn.ld + i] = acc[j * kLdsLd + i];
-  }
+  // write back; rows 0..nc-1 of a tile at row offset 0 are the panel's diagonal
+  // block, whose strict upper part stays untouched (zero)
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
+        if (i < nr && j < nc && !(tl.r0 == 0 && i < j)) {
+          gtile[(int64_t)j * pn.ld + i] = acc[bi][bj][r];
+          // this level factors the panel: the diagonal block also goes to the
+          // launch-invariant copy every factor-kernel tile of the panel reads
+          if (tl.kind == 0 && tl.slot >= 0 && i < nc) dscratch[tl.slot + j * nc + i] = acc[bi][bj][r];
+        }
+      }
   if (tl.kind == 0 && tl.slot >= 0) {
-    // this level factors the panel: hand the updated diagonal block to every
-    // factor-kernel tile through a copy that nobody overwrites in that launch
-    double* dcopy = dscratch + tl.slot;
     for (int e = tid; e < nc * nc; e += kThreads) {
       const int i = e % nc, j = e / nc;
-      dcopy[e] = i >= j ? acc[j * kLdsLd + i] : 0.0;
+      if (i < j) dscratch[tl.slot + e] = 0.0;
     }
   }
 }
@@ -272,7 +391,8 @@ constexpr int kFactThreads = 64;
 __global__ void __launch_bounds__(kFactThreads)
 k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
                const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
-               const double* __restrict__ dscratch, int* __restrict__ info_out) {
+               const double* __restrict__ dscratch, int* __restrict__ info_out,
+               float* __restrict__ Lf) {
   __shared__ double l11[swc::PW * kLdsLd];
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
@@ -311,11 +431,15 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
   for (int c = 0; c < swc::PW; ++c)
     if (lane < nc && c < nc) l11[c * kLdsLd + lane] = c <= lane ? x[c] : 0.0;
   double* gtile = L + pn.vbase + tl.r0;
+  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
   if (tl.kind == 0) {
     if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
 #pragma unroll
     for (int c = 0; c < swc::PW; ++c)
-      if (lane < nc && c <= lane) gtile[(int64_t)c * pn.ld + lane] = x[c];
+      if (lane < nc && c <= lane) {
+        gtile[(int64_t)c * pn.ld + lane] = x[c];
+        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+      }
   }
   // rows of this tile below the diagonal block: tile rows [first, nr)
   const int first = tl.kind == 0 ? nc : 0;
@@ -334,7 +458,10 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
   }
 #pragma unroll
   for (int c = 0; c < swc::PW; ++c)
-    if (mine && c < nc) gtile[(int64_t)c * pn.ld + lane] = x[c];
+    if (mine && c < nc) {
+      gtile[(int64_t)c * pn.ld + lane] = x[c];
+      if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+    }
 }
 
 // ---- triangular solves --------------------------------------------------
@@ -762,6 +889,7 @@ struct swc_handle {
   std::vector<int> pruneptr, pruneset;
   // device buffers
   double* dL = nullptr;
+  float* dLf = nullptr;  // FP32 shadow of L, written with it when FP32 updates are on
   int* dLs = nullptr;
   Panel* dPanels = nullptr;
   Tile* dTiles = nullptr;
@@ -838,14 +966,14 @@ struct swc_handle {
   }
   void release() {
     drop_graphs();
-    hipFree(dL); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
+    hipFree(dL); hipFree(dLf); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
     hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
     hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart); hipFree(dNormPart);
     hipFree(dRef);
     hipFree(dZ); hipFree(dArena); hipFree(dNodes); hipFree(dCol2sup); hipFree(dSpx);
     hipFree(dSsuper); hipFree(dSpi); hipFree(dDiagPos); hipFree(dQi); hipFree(dQj);
-    dL = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
+    dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
     dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
@@ -954,12 +1082,14 @@ void enqueue_factor(swc_handle* h, bool profile) {
     if (nt > 0)
       hipLaunchKernelGGL(k_tile_update, dim3(nt), dim3(kThreads), 0, st, h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0, h->fp32_mask, h->dDiag);
+                         (const Task*)h->dTasks, t0, h->fp32_mask, h->dDiag,
+                         (const float*)h->dLf);
     if (pe) hipEventRecord(pe[1], st);
     if (ns > 0)
       hipLaunchKernelGGL(k_panel_factor, dim3(ns), dim3(kFactThreads), 0, st, h->dL,
                          (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const int*)h->dTrsm, s0, (const double*)h->dDiag, h->dInfo);
+                         (const int*)h->dTrsm, s0, (const double*)h->dDiag, h->dInfo,
+                         h->fp32_mask ? h->dLf : nullptr);
     if (pe) hipEventRecord(pe[2], st);
   }
 }
@@ -1436,6 +1566,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   hipStream_t st = h->stream;
   int rc = SWC_OK;
   if (!rc) rc = upload(&h->dL, (const double*)nullptr, (size_t)P.xsize, st);
+  if (!rc && h->precision > 0) rc = upload(&h->dLf, (const float*)nullptr, (size_t)P.xsize, st);
   if (!rc) rc = upload(&h->dLs, h->Ls.data(), h->Ls.size(), st);
   if (!rc) rc = upload(&h->dPanels, P.panels.data(), P.panels.size(), st);
   if (!rc) rc = upload(&h->dTiles, P.tiles.data(), P.tiles.size(), st);
@@ -1603,6 +1734,8 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
 int swc_set_precision(swc_handle* h, int mode) {
   if (int rc = check_handle(h)) return rc;
   if (mode < 0 || mode > 2) return swc_set_error(SWC_EINVAL, "swc_set_precision: mode must be 0, 1 or 2");
+  if (mode > 0 && !h->dLf && h->plan.xsize > 0)
+    SWC_HIP(hipMalloc((void**)&h->dLf, sizeof(float) * (size_t)h->plan.xsize));
   h->precision = mode;
   h->fp32_mask = mode == 0 ? 0 : (mode == 1 ? 1 : 3);
   return SWC_OK;

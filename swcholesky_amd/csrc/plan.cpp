@@ -240,7 +240,18 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     pt.t.task_begin = (int)P.tasks.size();
     const Panel& pn = P.panels[pt.t.panel];
     const int* trow = Ls + pn.rows_off + pt.t.r0;
-    for (const Task& t : ptasks[oi]) {
+    for (Task t : ptasks[oi]) {
+      // contiguous landing: rows are a slice of the tile's rows, columns a run
+      const int* srows = Ls + t.rows_off;
+      const int M = t.rb - t.ra, N = t.cb - t.ca;
+      const int* tr = Ls + pn.rows_off + pt.t.r0;
+      const int p0 = int(std::lower_bound(tr, tr + pt.t.nr, srows[t.ra]) - tr);
+      bool rows_ok = p0 + M <= pt.t.nr;
+      for (int i = 0; rows_ok && i < M; ++i) rows_ok = tr[p0 + i] == srows[t.ra + i];
+      const bool cols_ok = srows[t.cb - 1] - srows[t.ca] == N - 1;
+      t.ti0 = rows_ok && cols_ok ? p0 : -1;
+      t.tj0 = rows_ok && cols_ok ? srows[t.ca] - pn.c0 : -1;
+      t.pad0 = t.pad1 = 0;
       P.tasks.push_back(t);
       P.update_flops += 2.0 * (t.rb - t.ra) * (double)(t.cb - t.ca) * t.k;
       // useful products: pairs (row, col) with row >= col (both sorted)
@@ -328,6 +339,16 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
 }
 
 void swc_plan_free(swc_plan* p) { delete p; }
+
+int swc_plan_struct_sizes(int64_t* out) {
+  if (!out) return swc_set_error(SWC_EINVAL, "swc_plan_struct_sizes: null argument");
+  out[0] = sizeof(swc::Panel);
+  out[1] = sizeof(swc::Tile);
+  out[2] = sizeof(swc::Task);
+  out[3] = sizeof(swc::SolveTask);
+  out[4] = sizeof(swc::SolveBlock);
+  return SWC_OK;
+}
 
 int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   if (!p || !out) return swc_set_error(SWC_EINVAL, "swc_plan_sizes: null argument");

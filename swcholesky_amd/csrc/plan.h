@@ -55,6 +55,9 @@ struct Task {
   int ra, rb;        // source rows landing in the tile's rows
   int ca, cb;        // source rows landing in the panel's columns
   int kind;          // 0: descendant supernode, 1: earlier panel of the same supernode
+  int ti0, tj0;      // >= 0: the rows land on tile rows ti0.. and the columns on panel
+                     // columns tj0.. contiguously (accumulated in registers); else -1
+  int pad0, pad1;
 };
 
 // Forward-solve gather into one panel from a descendant supernode d: rows
@@ -74,6 +77,12 @@ struct SolveBlock {
   int nr;
   int pad;
 };
+
+static_assert(sizeof(Panel) == 48, "Panel layout is part of the plan export");
+static_assert(sizeof(Tile) == 32, "Tile layout is part of the plan export");
+static_assert(sizeof(Task) == 56, "Task layout is part of the plan export");
+static_assert(sizeof(SolveTask) == 24, "SolveTask layout is part of the plan export");
+static_assert(sizeof(SolveBlock) == 16, "SolveBlock layout is part of the plan export");
 
 struct Plan {
   int n = 0, nsuper = 0, npanel = 0;

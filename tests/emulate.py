@@ -54,7 +54,7 @@ def factor(plan, Ls, xsize, amap, vals):
                 mask = trow[pr][:, None] >= (int(pn["c0"]) + pc)[None, :]
                 sub = acc[np.ix_(pr, pc)]
                 acc[np.ix_(pr, pc)] = np.where(mask, sub - C, sub)
-            keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["kind"] == 0 else \
+            keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["r0"] == 0 else \
                 np.ones((nr, nc), bool)
             L[idx[keep]] = acc[keep]
             if tl["slot"] >= 0 and tl["kind"] == 0:
