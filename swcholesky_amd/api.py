@@ -246,8 +246,7 @@ TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
                     ("ca", np.int32), ("cb", np.int32), ("kind", np.int32),
-                    ("ti0", np.int32), ("tj0", np.int32), ("pad0", np.int32),
-                    ("pad1", np.int32)])
+                    ("rmask", np.uint64), ("cmask", np.uint64)])
 STASK_DT = np.dtype([("voff", np.int64), ("rows_off", np.int32), ("ca", np.int32),
                      ("cb", np.int32), ("wd", np.int32)])
 SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("pad", np.int32)])

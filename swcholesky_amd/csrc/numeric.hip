@@ -72,395 +72,352 @@ __device__ inline int find_pos(const int* a, int len, int v) {
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 
-// One task's product C = A B^T, A = source rows [ra, ra+M), B = source rows
-// [ca, ca+N), inner dimension k (source columns, stride ld), on FP64 MFMA
-// (v_mfma_f64_16x16x4f64).  Wave w owns the 32x32 quadrant (w/2, w%2) of the
-// 64x64 output as 2x2 blocks of 16x16; lane l feeds A[l%16][k0+l/16] and
-// B^T[k0+l/16][l%16] and holds D[l/16 + 4r][l%16] in component r (rows of the
-// f64 variant are interleaved by 4, unlike the f32 variant's D[4(l/16)+r]; an
-// on-device probe of both builtins fixed these layouts).
-__device__ inline void task_mfma(const double* __restrict__ src, int ld, int k, int ra, int M,
-                                 int ca, int N, int wave, int lane, d4 (&c)[2][2]) {
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
-  if (wr >= M || wc >= N) return;  // wave-uniform: quadrant outside the task block
+// Lane layout of v_mfma_f64_16x16x4f64 on gfx950 (fixed by an on-device probe):
+// lane l feeds A[l%16][k0 + l/16] and B^T[k0 + l/16][l%16] and receives
+// D[l/16 + 4r][l%16] in component r.
+
+// Raw buffer resource over [base, base + bytes) (gfx9 descriptor word 3): a
+// buffer load whose byte offset falls outside returns zero.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+  const int n = (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, 0x00020000);
+}
+
+// The whole offset goes in the VGPR operand: the range check covers only the
+// VGPR offset (plus the instruction offset), not the SGPR one.
+__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+}
+__device__ inline float load_f32(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+}
+
+// Byte offset of the source row feeding tile position p (a tile row or a panel
+// column), elements of es bytes: the e-th set bit of mask receives source row
+// base + e.  Positions receiving nothing get an offset past every buffer, so
+// their loads read zero.
+__device__ inline unsigned landing_off(uint64_t mask, int p, int base, int es) {
+  return ((mask >> p) & 1ull) ? (unsigned)(base + __popcll(mask & ((1ull << p) - 1ull))) * es
+                              : 0x80000000u;
+}
+
+// acc += A B^T for one task, A = source rows landing on the tile's rows, B =
+// source rows landing on the panel's columns, both placed at their landing
+// positions, over this wave's share of the inner dimension: the 16-column
+// chunks starting at 16 w + 64 g.  acc[bi][bj] is the 16x16 block (bi, bj) of
+// the 64x64 tile in the f64 output layout.  A block is computed only if the
+// task lands rows in block row bi and columns in block column bj (uniform over
+// the wave) and, on the diagonal tile, bi >= bj (blocks above the diagonal are
+// never stored).  Operands come through buffer loads over the chunk's columns,
+// so rows that land nowhere and columns past k read as zero without branches.
+// With f32 the operands come from the FP32 copy of L (rounded to FP32) and the
+// products are accumulated in FP64 on the same MFMA.
+__device__ inline void task_update(const void* __restrict__ src, bool f32, int ld, int k, int ra,
+                                   int ca, uint64_t rmask, uint64_t cmask, bool diag, int wave,
+                                   int lane, d4 (&acc)[4][4]) {
   const int lr = lane & 15, lk = lane >> 4;
-  const int r0 = wr + lr, r1 = wr + 16 + lr;
-  const int q0 = wc + lr, q1 = wc + 16 + lr;
-  const bool a0ok = r0 < M, a1ok = r1 < M, b0ok = q0 < N, b1ok = q1 < N;
-  const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;  // wave-uniform
-  const double* pa = src + ra;
-  const double* pb = src + ca;
-  // 16 k per iteration: all operand loads are issued before the 16 MFMAs
-  for (int kk = 0; kk < k; kk += 16) {
-    double a0[4], a1[4], b0[4], b1[4];
+  const int es = f32 ? 4 : 8;
+  const unsigned kstep = (unsigned)ld * es;  // bytes between source columns
+  unsigned ao[4], bo[4];
+  bool ab[4], bb[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kc = kk + 4 * u + lk;
-      const bool kok = kc < k;
-      const int64_t off = (int64_t)kc * ld;
-      a0[u] = (a0ok && kok) ? pa[off + r0] : 0.0;
-      a1[u] = (a1ok && kok) ? pa[off + r1] : 0.0;
-      b0[u] = (b0ok && kok) ? pb[off + q0] : 0.0;
-      b1[u] = (b1ok && kok) ? pb[off + q1] : 0.0;
+  for (int b = 0; b < 4; ++b) {
+    ao[b] = landing_off(rmask, 16 * b + lr, ra, es) + lk * kstep;
+    bo[b] = landing_off(cmask, 16 * b + lr, ca, es) + lk * kstep;
+    ab[b] = ((rmask >> (16 * b)) & 0xffffull) != 0;
+    bb[b] = ((cmask >> (16 * b)) & 0xffffull) != 0;
+  }
+  for (int kb = 16 * wave; kb < k; kb += 64) {
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc((const char*)src + (int64_t)kb * kstep,
+                                                (int64_t)(k - kb) * kstep);
+    double a[4][4], bv[4][4];
+    if (f32) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          a[b][u] = ab[b] ? (double)load_f32(rs, ao[b] + 4 * u * kstep) : 0.0;
+          bv[b][u] = bb[b] ? (double)load_f32(rs, bo[b] + 4 * u * kstep) : 0.0;
+        }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          a[b][u] = ab[b] ? load_f64(rs, ao[b] + 4 * u * kstep) : 0.0;
+          bv[b][u] = bb[b] ? load_f64(rs, bo[b] + 4 * u * kstep) : 0.0;
+        }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], c[0][0], 0, 0, 0);
-      if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], c[0][1], 0, 0, 0);
-      if (blk_a1) {
-        c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], c[1][0], 0, 0, 0);
-        if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], c[1][1], 0, 0, 0);
-      }
-    }
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+        if (ab[bi] && bb[bj] && !(diag && bi < bj)) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][u], bv[bj][u], acc[bi][bj], 0, 0, 0);
+        }
   }
 }
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// The same product with operands rounded to FP32 and FP32 accumulation
-// (v_mfma_f32_16x16x4f32, D[4*(l/16)+r][l%16] in component r); the caller
-// subtracts the FP32 result from its FP64 accumulator.
-__device__ inline void task_mfma_f32(const float* __restrict__ src, int ld, int k, int ra, int M,
-                                     int ca, int N, int wave, int lane, f4 (&c)[2][2]) {
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+// acc (f64 MFMA layout) <-> a 64x64 LDS tile (column j at buf[j * kLdsLd])
+__device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
-  if (wr >= M || wc >= N) return;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int r0 = wr + lr, r1 = wr + 16 + lr;
-  const int q0 = wc + lr, q1 = wc + 16 + lr;
-  const bool a0ok = r0 < M, a1ok = r1 < M, b0ok = q0 < N, b1ok = q1 < N;
-  const bool blk_a1 = wr + 16 < M, blk_b1 = wc + 16 < N;
-  const float* pa = src + ra;
-  const float* pb = src + ca;
-  for (int kk = 0; kk < k; kk += 16) {
-    float a0[4], a1[4], b0[4], b1[4];
+    for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kc = kk + 4 * u + lk;
-      const bool kok = kc < k;
-      const int64_t off = (int64_t)kc * ld;
-      a0[u] = (a0ok && kok) ? pa[off + r0] : 0.f;
-      a1[u] = (a1ok && kok) ? pa[off + r1] : 0.f;
-      b0[u] = (b0ok && kok) ? pb[off + q0] : 0.f;
-      b1[u] = (b1ok && kok) ? pb[off + q1] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], c[0][0], 0, 0, 0);
-      if (blk_b1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], c[0][1], 0, 0, 0);
-      if (blk_a1) {
-        c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], c[1][0], 0, 0, 0);
-        if (blk_b1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], c[1][1], 0, 0, 0);
-      }
-    }
-  }
+      for (int r = 0; r < 4; ++r)
+        buf[(16 * bj + (lane & 15)) * kLdsLd + 16 * bi + (lane >> 4) + 4 * r] = acc[bi][bj][r];
 }
 
-// Contiguous task on FP64 MFMA straight into the tile accumulator: acc -= A B^T
-// where tile row tr takes source row ra + tr - ti0 and tile column tc takes
-// source row ca + tc - tj0 (rows/columns outside the task block read as zero).
-__device__ inline void dense_mfma_f64(const double* __restrict__ src, int ld, int k, int ra, int M,
-                                      int ca, int N, int ti0, int tj0, int wave, int lane,
-                                      d4 (&acc)[2][2]) {
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  if (wr + 32 <= ti0 || wr >= ti0 + M || wc + 32 <= tj0 || wc >= tj0 + N) return;  // uniform
-  const int lr = lane & 15, lk = lane >> 4;
-  const int sa0 = wr + lr - ti0, sa1 = sa0 + 16;      // source row offsets of the A operand
-  const int sb0 = wc + lr - tj0, sb1 = sb0 + 16;
-  const bool a0ok = sa0 >= 0 && sa0 < M, a1ok = sa1 >= 0 && sa1 < M;
-  const bool b0ok = sb0 >= 0 && sb0 < N, b1ok = sb1 >= 0 && sb1 < N;
-  const bool ba0 = wr + 16 > ti0 && wr < ti0 + M, ba1 = wr + 32 > ti0 && wr + 16 < ti0 + M;
-  const bool bb0 = wc + 16 > tj0 && wc < tj0 + N, bb1 = wc + 32 > tj0 && wc + 16 < tj0 + N;
-  const double* pa = src + ra;
-  const double* pb = src + ca;
-  for (int kk = 0; kk < k; kk += 16) {
-    double a0[4], a1[4], b0[4], b1[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kc = kk + 4 * u + lk;
-      const bool kok = kc < k;
-      const int64_t off = (int64_t)kc * ld;
-      a0[u] = (a0ok && kok) ? -pa[off + sa0] : 0.0;
-      a1[u] = (a1ok && kok) ? -pa[off + sa1] : 0.0;
-      b0[u] = (b0ok && kok) ? pb[off + sb0] : 0.0;
-      b1[u] = (b1ok && kok) ? pb[off + sb1] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (ba0 && bb0) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0][0], 0, 0, 0);
-      if (ba0 && bb1) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[0][1], 0, 0, 0);
-      if (ba1 && bb0) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[1][0], 0, 0, 0);
-      if (ba1 && bb1) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[1][1], 0, 0, 0);
-    }
-  }
-}
-
-// The same on FP32 MFMA with FP32 accumulation, subtracted into the FP64
-// accumulator.  The A operand rows are fed in the order pi(r) = r/4 + 4(r%4),
-// which makes the FP32 output land in the FP64 variant's register layout.
-__device__ inline void dense_mfma_f32(const float* __restrict__ src, int ld, int k, int ra, int M,
-                                      int ca, int N, int ti0, int tj0, int wave, int lane,
-                                      d4 (&acc)[2][2]) {
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  if (wr + 32 <= ti0 || wr >= ti0 + M || wc + 32 <= tj0 || wc >= tj0 + N) return;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int pr_ = (lr >> 2) + 4 * (lr & 3);
-  const int sa0 = wr + pr_ - ti0, sa1 = sa0 + 16;
-  const int sb0 = wc + lr - tj0, sb1 = sb0 + 16;
-  const bool a0ok = sa0 >= 0 && sa0 < M, a1ok = sa1 >= 0 && sa1 < M;
-  const bool b0ok = sb0 >= 0 && sb0 < N, b1ok = sb1 >= 0 && sb1 < N;
-  const bool ba0 = wr + 16 > ti0 && wr < ti0 + M, ba1 = wr + 32 > ti0 && wr + 16 < ti0 + M;
-  const bool bb0 = wc + 16 > tj0 && wc < tj0 + N, bb1 = wc + 32 > tj0 && wc + 16 < tj0 + N;
-  const float* pa = src + ra;
-  const float* pb = src + ca;
-  f4 c[2][2];
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) c[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int kk = 0; kk < k; kk += 16) {
-    float a0[4], a1[4], b0[4], b1[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kc = kk + 4 * u + lk;
-      const bool kok = kc < k;
-      const int64_t off = (int64_t)kc * ld;
-      a0[u] = (a0ok && kok) ? pa[off + sa0] : 0.f;
-      a1[u] = (a1ok && kok) ? pa[off + sa1] : 0.f;
-      b0[u] = (b0ok && kok) ? pb[off + sb0] : 0.f;
-      b1[u] = (b1ok && kok) ? pb[off + sb1] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (ba0 && bb0) c[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], c[0][0], 0, 0, 0);
-      if (ba0 && bb1) c[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], c[0][1], 0, 0, 0);
-      if (ba1 && bb0) c[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], c[1][0], 0, 0, 0);
-      if (ba1 && bb1) c[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], c[1][1], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[bi][bj][r] -= (double)c[bi][bj][r];
-}
-
-// Tile kernel: applies all source updates of one tile, in task order, to an
-// accumulator held in MFMA registers (lane l, component r of block (bi, bj) of
-// wave w is tile element (wr + 16 bi + l/16 + 4 r, wc + 16 bj + l%16)), then
-// writes the tile back.  Contiguous tasks multiply straight into the registers;
-// a scattered task computes its block, scatters it through an LDS staging tile
-// at its mapped rows and columns, and every lane pulls back its own elements.
-// The factorization of panels completed by this level follows in k_panel_factor.
-__global__ void __launch_bounds__(kThreads)
-k_tile_update(double* __restrict__ L, const int* __restrict__ Ls,
-              const Panel* __restrict__ panels, const Tile* __restrict__ tiles,
-              const Task* __restrict__ tasks, int tile0, int fp32_mask,
-              double* __restrict__ dscratch, const float* __restrict__ Lf) {
-  __shared__ double stage[swc::PW * kLdsLd];
-  __shared__ int trow[swc::TM];
-  __shared__ int pr[swc::TM];
-  __shared__ int pc[swc::TM];
+// Tile kernel: applies all source updates of one tile, in task order.  Every
+// wave accumulates the products of a fixed share of each task's inner
+// dimension over the whole 64x64 tile in MFMA registers; the four partial sums
+// are combined in a fixed order, (w0 + w1) + (w2 + w3), through LDS and
+// subtracted from the tile, so results do not depend on scheduling.  The
+// factorization of panels completed by this level follows in k_panel_factor.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
+              const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
+              int fp32_mask, double* __restrict__ dscratch, const float* __restrict__ Lf) {
+  __shared__ double red[2][swc::PW * kLdsLd];
   const Tile tl = tiles[tile0 + blockIdx.x];
   const Panel pn = panels[tl.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = tl.nr, nc = pn.ncol;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const bool diag = tl.r0 == 0;
   double* gtile = L + pn.vbase + tl.r0;
-  d4 acc[2][2];
+  d4 acc[4][4];
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
-        acc[bi][bj][r] = (i < nr && j < nc) ? gtile[(int64_t)j * pn.ld + i] : 0.0;
-      }
-  bool have_rows = false;
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
   for (int t = tl.task_begin; t < tl.task_end; ++t) {
     const Task tk = tasks[t];
-    const int M = tk.rb - tk.ra, N = tk.cb - tk.ca;
-    const double* src = L + tk.src_base;
     const bool f32 = (fp32_mask >> tk.kind) & 1;
-    if (tk.ti0 >= 0) {
-      if (f32) dense_mfma_f32(Lf + tk.src_base, tk.ld, tk.k, tk.ra, M, tk.ca, N, tk.ti0, tk.tj0, wave, lane, acc);
-      else dense_mfma_f64(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, tk.ti0, tk.tj0, wave, lane, acc);
-      continue;
-    }
-    // scattered task: map its rows and columns, stage its block, pull it back
-    if (!have_rows) {
-      for (int e = tid; e < nr; e += kThreads) trow[e] = Ls[pn.rows_off + tl.r0 + e];
-      have_rows = true;
-    }
-    for (int e = tid; e < swc::PW * kLdsLd; e += kThreads) stage[e] = 0.0;
-    for (int e = tid; e < M; e += kThreads) pr[e] = find_pos(trow, nr, Ls[tk.rows_off + tk.ra + e]);
-    for (int e = tid; e < N; e += kThreads) pc[e] = Ls[tk.rows_off + tk.ca + e] - pn.c0;
-    __syncthreads();
-    if (tk.k < 8) {
-      for (int e = tid; e < M * N; e += kThreads) {
-        const int i = e % M, j = e / M;
-        const double* a = src + tk.ra + i;
-        const double* b = src + tk.ca + j;
-        double sacc = 0.0;
-        for (int kk = 0; kk < tk.k; ++kk) sacc = fma(a[(int64_t)kk * tk.ld], b[(int64_t)kk * tk.ld], sacc);
-        stage[pc[j] * kLdsLd + pr[i]] = sacc;
-      }
-    } else if (f32) {
-      f4 c[2][2];
-      task_mfma_f32(Lf + tk.src_base, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = wr + 16 * bi + 4 * (lane >> 4) + r;
-            const int j = wc + 16 * bj + (lane & 15);
-            if (i < M && j < N) stage[pc[j] * kLdsLd + pr[i]] = (double)c[bi][bj][r];
-          }
-    } else {
-      d4 c[2][2];
-      task_mfma(src, tk.ld, tk.k, tk.ra, M, tk.ca, N, wave, lane, c);
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = wr + 16 * bi + (lane >> 4) + 4 * r;
-            const int j = wc + 16 * bj + (lane & 15);
-            if (i < M && j < N) stage[pc[j] * kLdsLd + pr[i]] = c[bi][bj][r];
-          }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
-          acc[bi][bj][r] -= stage[j * kLdsLd + i];
-        }
-    __syncthreads();
+    const void* src = f32 ? (const void*)(Lf + tk.src_base) : (const void*)(L + tk.src_base);
+    task_update(src, f32, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask, diag, wave, lane, acc);
   }
-  // write back; rows 0..nc-1 of a tile at row offset 0 are the panel's diagonal
-  // block, whose strict upper part stays untouched (zero)
+  // partial sums (w0 + w1) and (w2 + w3), then their sum, element-wise by all
+  // threads: thread t owns elements t, t + 256, ... of the column-major tile
+  double s01[16];
+  if (wave < 2) acc_store(acc, red[wave], lane);
+  __syncthreads();
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
+    s01[q] = red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i];
+  }
+  __syncthreads();
+  if (wave >= 2) acc_store(acc, red[wave - 2], lane);
+  __syncthreads();
+  // write back tile - sum; rows 0..nc-1 of a tile at row offset 0 are the
+  // panel's diagonal block, whose strict upper part stays untouched (zero).
+  // When this level factors the panel, the diagonal block also goes to the
+  // launch-invariant copy that every factor-kernel tile of the panel reads.
+  const bool copy = tl.kind == 0 && tl.slot >= 0;
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wr + 16 * bi + (lane >> 4) + 4 * r, j = wc + 16 * bj + (lane & 15);
-        if (i < nr && j < nc && !(tl.r0 == 0 && i < j)) {
-          gtile[(int64_t)j * pn.ld + i] = acc[bi][bj][r];
-          // this level factors the panel: the diagonal block also goes to the
-          // launch-invariant copy every factor-kernel tile of the panel reads
-          if (tl.kind == 0 && tl.slot >= 0 && i < nc) dscratch[tl.slot + j * nc + i] = acc[bi][bj][r];
-        }
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
+    if (i < nr && j < nc) {
+      const double s23 = red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i];
+      if (!(diag && i < j)) {
+        const double v = gtile[(int64_t)j * pn.ld + i] - (s01[q] + s23);
+        gtile[(int64_t)j * pn.ld + i] = v;
+        if (copy && i < nc) dscratch[tl.slot + j * nc + i] = v;
+      } else if (copy) {
+        dscratch[tl.slot + j * nc + i] = 0.0;
       }
-  if (tl.kind == 0 && tl.slot >= 0) {
-    for (int e = tid; e < nc * nc; e += kThreads) {
-      const int i = e % nc, j = e / nc;
-      if (i < j) dscratch[tl.slot + e] = 0.0;
     }
   }
 }
 
-// Panel factorization, one wave per tile of a panel completed at this level:
-// POTRF of the panel's nc x nc diagonal block (dpotrf 'L',
-// sw_parallel_PB_Cholesky_05.c:430) in registers -- lane i holds row i, column
-// j's multipliers are broadcast by shuffles -- then B <- B L11^{-T} for the
-// tile's rows below the diagonal block (dtrsm 'R','L','T','N', .c:461-467),
-// lane i substituting its own row against L11 staged in LDS.  Every tile of a
-// panel computes the same L11 from the same inputs (bitwise identical), so no
-// tile waits for another; the diagonal tile stores it and reports info.
-constexpr int kFactThreads = 64;
+// Broadcast of lane src's value (src uniform over the wave).
+__device__ inline double bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
 
-__global__ void __launch_bounds__(kFactThreads)
+// Panel factorization, one wave per tile of a panel completed at this level,
+// for panels of at most NC (16 or 32) columns: POTRF of the panel's nc x nc
+// diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430) in registers --
+// lane i holds row i, the pivot and the multipliers of column j are read from
+// their lanes -- then B <- B L11^{-T} for the tile's rows below the diagonal
+// block (dtrsm 'R','L','T','N', .c:461-467), lane i substituting its own row.
+// Both apply the updates of an entry in increasing column order, as the
+// left-looking loops of the reference do.  Every tile of a panel computes the
+// same L11 from the same launch-invariant copy (bitwise identical), so no tile
+// waits for another; the diagonal tile stores it and reports info.  Entries
+// above the diagonal of the register copy are never read by lanes on or below
+// it, so they are left unmasked.
+template <int NC>
+__global__ void __launch_bounds__(64)
 k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
                const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
                const double* __restrict__ dscratch, int* __restrict__ info_out,
                float* __restrict__ Lf) {
-  __shared__ double l11[swc::PW * kLdsLd];
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
   const int lane = threadIdx.x;
   const int nr = tl.nr, nc = pn.ncol;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
-  __shared__ double colj[swc::PW];  // column j of L11 during the factorization
-  double x[swc::PW];
+  double y[NC];                              // row `lane` of the diagonal block
 #pragma unroll
-  for (int c = 0; c < swc::PW; ++c) x[c] = (lane < nc && c <= lane) ? gdiag[c * nc + lane] : 0.0;
+  for (int c = 0; c < NC; ++c) y[c] = (lane < nc && c < nc && c <= lane) ? gdiag[c * nc + lane] : 0.0;
   int badcol = 0;
 #pragma unroll
-  for (int j = 0; j < swc::PW; ++j) {
+  for (int j = 0; j < NC; ++j) {
     if (j < nc) {
-      // the pivot lives in lane j; publish it, then the scaled column, via LDS
-      if (lane == j) colj[j] = x[j];
-      __syncthreads();
-      const double d = colj[j];
+      const double d = bcast(y[j], j);
       const bool ok = d > 0.0;
       if (!ok && badcol == 0) badcol = j + 1;
       const double sq = ok ? sqrt(d) : 1.0;  // keep going without NaNs; info reports it
-      if (lane == j) x[j] = sq;
-      else if (lane > j) x[j] /= sq;
-      __syncthreads();
-      if (lane < nc) colj[lane] = x[j];
-      __syncthreads();
-      const double lij = x[j];
+      if (lane == j) y[j] = sq;
+      else if (lane > j) y[j] /= sq;
 #pragma unroll
-      for (int c = j + 1; c < swc::PW; ++c)
-        if (c < nc && lane >= c) x[c] -= lij * colj[c];
+      for (int c = j + 1; c < NC; ++c)
+        if (c < nc) y[c] -= y[j] * bcast(y[j], c);
     }
   }
-  __syncthreads();
-  // stage L11 (row `lane` of it) in LDS
-#pragma unroll
-  for (int c = 0; c < swc::PW; ++c)
-    if (lane < nc && c < nc) l11[c * kLdsLd + lane] = c <= lane ? x[c] : 0.0;
   double* gtile = L + pn.vbase + tl.r0;
   float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
   if (tl.kind == 0) {
     if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
 #pragma unroll
-    for (int c = 0; c < swc::PW; ++c)
+    for (int c = 0; c < NC; ++c)
       if (lane < nc && c <= lane) {
-        gtile[(int64_t)c * pn.ld + lane] = x[c];
-        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+        gtile[(int64_t)c * pn.ld + lane] = y[c];
+        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)y[c];
       }
   }
   // rows of this tile below the diagonal block: tile rows [first, nr)
   const int first = tl.kind == 0 ? nc : 0;
   const bool mine = lane >= first && lane < nr;
+  double x[NC];
 #pragma unroll
-  for (int c = 0; c < swc::PW; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
-  __syncthreads();
+  for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
 #pragma unroll
-  for (int j = 0; j < swc::PW; ++j) {
+  for (int j = 0; j < NC; ++j) {
     if (j < nc) {
-      x[j] *= 1.0 / l11[j * kLdsLd + j];
+      x[j] *= 1.0 / bcast(y[j], j);
 #pragma unroll
-      for (int c = j + 1; c < swc::PW; ++c)
-        if (c < nc) x[c] -= x[j] * l11[j * kLdsLd + c];
+      for (int c = j + 1; c < NC; ++c)
+        if (c < nc) x[c] -= x[j] * bcast(y[j], c);
     }
   }
 #pragma unroll
-  for (int c = 0; c < swc::PW; ++c)
+  for (int c = 0; c < NC; ++c)
     if (mine && c < nc) {
       gtile[(int64_t)c * pn.ld + lane] = x[c];
       if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+    }
+}
+
+// The same for panels of 33..64 columns, four waves per tile: wave w holds
+// columns [16w, 16w+16) of row `lane` (the diagonal block is padded to 64 with
+// the identity, the tile rows with zeros).  POTRF and the substitution both go
+// block column by block column: wave J factors (resp. solves) its 16 columns
+// from its registers and publishes them in LDS, then the waves to its right
+// subtract that block's products from their columns, in increasing column
+// order as before.  Per tile this is 8 dependent phases of at most 16x16
+// operations per lane instead of 64 sequential columns on one wave.
+__global__ void __launch_bounds__(256)
+k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
+                 const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
+                 const double* __restrict__ dscratch, int* __restrict__ info_out,
+                 float* __restrict__ Lf) {
+  __shared__ double sl[64 * kLdsLd];  // L11: column c at sl[c * kLdsLd]
+  __shared__ double sx[64 * kLdsLd];  // solved tile columns, same layout
+  const Tile tl = tiles[list[list0 + blockIdx.x]];
+  const Panel pn = panels[tl.panel];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nr = tl.nr, nc = pn.ncol;
+  const int c0 = 16 * w;  // first column of this wave
+  const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
+  double y[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const int col = c0 + c;
+    y[c] = (lane < nc && col < nc) ? (col <= lane ? gdiag[col * nc + lane] : 0.0)
+                                   : (col == lane ? 1.0 : 0.0);
+  }
+  int badcol = 0;
+  for (int J = 0; J < 4; ++J) {
+    if (w == J) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const int jg = c0 + jj;
+        const double d = bcast(y[jj], jg);
+        const bool ok = d > 0.0;
+        badcol = (!ok && badcol == 0) ? jg + 1 : badcol;
+        const double sq = ok ? sqrt(d) : 1.0;  // keep going without NaNs; info reports it
+        y[jj] = lane == jg ? sq : (lane > jg ? y[jj] / sq : y[jj]);
+#pragma unroll
+        for (int c = jj + 1; c < 16; ++c) y[c] -= y[jj] * bcast(y[jj], c0 + c);
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) sl[(c0 + c) * kLdsLd + lane] = y[c];
+    }
+    __syncthreads();
+    if (w > J) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const double* lj = sl + (16 * J + jj) * kLdsLd;
+        const double lij = lj[lane];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) y[c] -= lij * lj[c0 + c];
+      }
+    }
+  }
+  double* gtile = L + pn.vbase + tl.r0;
+  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
+  if (tl.kind == 0) {
+    if (badcol && badcol <= nc && lane == 0) atomicMin(info_out, pn.c0 + badcol);
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (lane < nc && c0 + c < nc && c0 + c <= lane) {
+        gtile[(int64_t)(c0 + c) * pn.ld + lane] = y[c];
+        if (ftile) ftile[(int64_t)(c0 + c) * pn.ld + lane] = (float)y[c];
+      }
+  }
+  // rows of this tile below the diagonal block: tile rows [first, nr)
+  const int first = tl.kind == 0 ? nc : 0;
+  const bool mine = lane >= first && lane < nr;
+  double x[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) x[c] = (mine && c0 + c < nc) ? gtile[(int64_t)(c0 + c) * pn.ld + lane] : 0.0;
+  for (int J = 0; J < 4; ++J) {
+    if (w == J) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const double* lj = sl + (c0 + jj) * kLdsLd;
+        x[jj] *= 1.0 / lj[c0 + jj];
+#pragma unroll
+        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c0 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
+    }
+    __syncthreads();
+    if (w > J) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const double* lj = sl + (16 * J + jj) * kLdsLd;
+        const double xj = sx[(16 * J + jj) * kLdsLd + lane];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] -= xj * lj[c0 + c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+    if (mine && c0 + c < nc) {
+      gtile[(int64_t)(c0 + c) * pn.ld + lane] = x[c];
+      if (ftile) ftile[(int64_t)(c0 + c) * pn.ld + lane] = (float)x[c];
     }
 }
 
@@ -1081,15 +1038,27 @@ void enqueue_factor(swc_handle* h, bool profile) {
     if (pe) hipEventRecord(pe[0], st);
     if (nt > 0)
       hipLaunchKernelGGL(k_tile_update, dim3(nt), dim3(kThreads), 0, st, h->dL,
-                         (const int*)h->dLs, (const Panel*)h->dPanels, (const Tile*)h->dTiles,
+                         (const Panel*)h->dPanels, (const Tile*)h->dTiles,
                          (const Task*)h->dTasks, t0, h->fp32_mask, h->dDiag,
                          (const float*)h->dLf);
     if (pe) hipEventRecord(pe[1], st);
-    if (ns > 0)
-      hipLaunchKernelGGL(k_panel_factor, dim3(ns), dim3(kFactThreads), 0, st, h->dL,
-                         (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const int*)h->dTrsm, s0, (const double*)h->dDiag, h->dInfo,
-                         h->fp32_mask ? h->dLf : nullptr);
+    if (ns > 0) {
+      // the level's panels by width class: <= 16, <= 32, <= 64 columns
+      const int* cls = &P.lvl_trsm_cls[(size_t)4 * l];
+      float* lf = h->fp32_mask ? h->dLf : nullptr;
+      const Panel* pp = (const Panel*)h->dPanels;
+      const Tile* tp = (const Tile*)h->dTiles;
+      const int* lp = (const int*)h->dTrsm;
+      if (cls[1] > cls[0])
+        hipLaunchKernelGGL(k_panel_factor<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
+                           lp, cls[0], (const double*)h->dDiag, h->dInfo, lf);
+      if (cls[2] > cls[1])
+        hipLaunchKernelGGL(k_panel_factor<32>, dim3(cls[2] - cls[1]), dim3(64), 0, st, h->dL, pp, tp,
+                           lp, cls[1], (const double*)h->dDiag, h->dInfo, lf);
+      if (cls[3] > cls[2])
+        hipLaunchKernelGGL(k_panel_factor64, dim3(cls[3] - cls[2]), dim3(kThreads), 0, st, h->dL, pp,
+                           tp, lp, cls[2], (const double*)h->dDiag, h->dInfo, lf);
+    }
     if (pe) hipEventRecord(pe[2], st);
   }
 }

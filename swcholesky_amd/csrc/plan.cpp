@@ -241,17 +241,20 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     const Panel& pn = P.panels[pt.t.panel];
     const int* trow = Ls + pn.rows_off + pt.t.r0;
     for (Task t : ptasks[oi]) {
-      // contiguous landing: rows are a slice of the tile's rows, columns a run
+      // landing masks: source rows merged against the tile's sorted rows
       const int* srows = Ls + t.rows_off;
-      const int M = t.rb - t.ra, N = t.cb - t.ca;
-      const int* tr = Ls + pn.rows_off + pt.t.r0;
-      const int p0 = int(std::lower_bound(tr, tr + pt.t.nr, srows[t.ra]) - tr);
-      bool rows_ok = p0 + M <= pt.t.nr;
-      for (int i = 0; rows_ok && i < M; ++i) rows_ok = tr[p0 + i] == srows[t.ra + i];
-      const bool cols_ok = srows[t.cb - 1] - srows[t.ca] == N - 1;
-      t.ti0 = rows_ok && cols_ok ? p0 : -1;
-      t.tj0 = rows_ok && cols_ok ? srows[t.ca] - pn.c0 : -1;
-      t.pad0 = t.pad1 = 0;
+      t.rmask = 0;
+      for (int e = t.ra, i = 0; e < t.rb; ++e) {
+        while (i < pt.t.nr && trow[i] < srows[e]) ++i;
+        if (i == pt.t.nr || trow[i] != srows[e]) throw std::logic_error("source row not in tile");
+        t.rmask |= 1ull << i;
+      }
+      t.cmask = 0;
+      for (int e = t.ca; e < t.cb; ++e) {
+        const int j = srows[e] - pn.c0;
+        if (j < 0 || j >= pn.ncol) throw std::logic_error("source row not in panel");
+        t.cmask |= 1ull << j;
+      }
       P.tasks.push_back(t);
       P.update_flops += 2.0 * (t.rb - t.ra) * (double)(t.cb - t.ca) * t.k;
       // useful products: pairs (row, col) with row >= col (both sorted)
@@ -263,7 +266,6 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       }
       P.tile_kernel_flops += 2.0 * pairs * t.k;
     }
-    (void)trow;
     pt.t.task_end = (int)P.tasks.size();
     P.lvl_tiles[pt.level + 1]++;
     if (pt.trsm || pt.t.kind == 0) trsm_by_level[pt.level].push_back((int)P.tiles.size());
@@ -297,8 +299,25 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     if (off > 0x7fffffff) throw std::invalid_argument("diagonal scratch too large");
     P.diag_scratch = std::max(P.diag_scratch, off);
   }
+  // each level's factor list grouped by panel width class (one launch per class)
+  P.lvl_trsm_cls.assign((size_t)4 * P.nlevel, 0);
+  auto width_class = [&](int ti) {
+    const int nc = P.panels[P.tiles[ti].panel].ncol;
+    return nc <= 16 ? 0 : (nc <= 32 ? 1 : 2);
+  };
   for (int l = 0; l < P.nlevel; ++l) {
-    P.trsm_tiles.insert(P.trsm_tiles.end(), trsm_by_level[l].begin(), trsm_by_level[l].end());
+    std::vector<int>& v = trsm_by_level[l];
+    std::stable_sort(v.begin(), v.end(),
+                     [&](int a, int b) { return width_class(a) < width_class(b); });
+    int* cls = &P.lvl_trsm_cls[(size_t)4 * l];
+    const int base = (int)P.trsm_tiles.size();
+    int counts[3] = {0, 0, 0};
+    for (int ti : v) counts[width_class(ti)]++;
+    cls[0] = base;
+    cls[1] = cls[0] + counts[0];
+    cls[2] = cls[1] + counts[1];
+    cls[3] = cls[2] + counts[2];
+    P.trsm_tiles.insert(P.trsm_tiles.end(), v.begin(), v.end());
     P.lvl_trsm[l + 1] = (int)P.trsm_tiles.size();
   }
   for (const Panel& pn : P.panels) {

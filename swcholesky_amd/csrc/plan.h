@@ -46,7 +46,10 @@ struct Tile {
 };
 
 // One source contribution to one tile: rows [ra,rb) x columns [ca,cb) of the
-// source's row list, inner dimension k.
+// source's row list, inner dimension k.  Where they land is two bit masks: the
+// e-th set bit of rmask is the tile row that source row ra+e lands on, the e-th
+// set bit of cmask the panel column of source row ca+e (both lists are sorted,
+// so the landing positions are increasing).
 struct Task {
   int64_t src_base;  // value offset of (source row 0, source column 0)
   int ld;            // source leading dimension
@@ -55,9 +58,8 @@ struct Task {
   int ra, rb;        // source rows landing in the tile's rows
   int ca, cb;        // source rows landing in the panel's columns
   int kind;          // 0: descendant supernode, 1: earlier panel of the same supernode
-  int ti0, tj0;      // >= 0: the rows land on tile rows ti0.. and the columns on panel
-                     // columns tj0.. contiguously (accumulated in registers); else -1
-  int pad0, pad1;
+  uint64_t rmask;    // tile rows receiving source rows ra..rb-1
+  uint64_t cmask;    // panel columns receiving source rows ca..cb-1
 };
 
 // Forward-solve gather into one panel from a descendant supernode d: rows
@@ -92,6 +94,8 @@ struct Plan {
   std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
   std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)
   std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles
+  std::vector<int> lvl_trsm_cls;    // 4 per level: starts of the panel-width classes
+                                    // (<= 16, <= 32, <= 64 columns) in trsm_tiles, and the end
   std::vector<Task> tasks;
   std::vector<int> panel_level;     // per panel
   std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order

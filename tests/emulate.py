@@ -1,7 +1,7 @@
 """Numpy emulation of the GPU kernels on the real device plan (test infrastructure).
 
-Runs the same index arithmetic as k_tile_update_factor / k_tile_trsm /
-k_fwd_panel / k_bwd_panel in swcholesky_amd/csrc/numeric.hip, asserting that every
+Runs the same index arithmetic as k_tile_update / k_panel_factor / k_fwd_diag /
+k_fwd_update / k_bwd_partial / k_bwd_diag in swcholesky_amd/csrc/numeric.hip, asserting that every
 index stays inside its buffer, so plan and kernel indexing are checked on the CPU.
 """
 import numpy as np
@@ -9,6 +9,10 @@ import numpy as np
 
 def _chk(lo, hi, size, what):
     assert 0 <= lo and hi <= size, f"{what}: [{lo},{hi}) outside [0,{size})"
+
+
+def _bits(m):
+    return np.array([i for i in range(64) if (m >> i) & 1], dtype=np.int64)
 
 
 def factor(plan, Ls, xsize, amap, vals):
@@ -39,21 +43,21 @@ def factor(plan, Ls, xsize, amap, vals):
                 k, sld, sb = int(tk["k"]), int(tk["ld"]), int(tk["src_base"])
                 roff = int(tk["rows_off"])
                 assert ra < rb and ca < cb and rb - ra <= 64 and cb - ca <= 64
+                # landing positions: the e-th set bit receives source row ra+e / ca+e
+                pr = _bits(int(tk["rmask"]))
+                pc = _bits(int(tk["cmask"]))
+                assert len(pr) == rb - ra and len(pc) == cb - ca, "mask/row count mismatch"
+                assert pr.max() < nr and pc.max() < nc, "landing outside the tile"
                 _chk(roff + min(ra, ca), roff + max(rb, cb), len(Ls), "task rows")
-                _chk(sb + min(ra, ca), sb + (k - 1) * sld + max(rb, cb), xsize, "task src")
-                rv = Ls[roff + ra:roff + rb]
-                cv = Ls[roff + ca:roff + cb]
-                pr = np.searchsorted(trow, rv)
-                assert np.all(pr < nr) and np.all(trow[pr] == rv), "row not in tile"
-                pc = cv - int(pn["c0"])
-                assert np.all((pc >= 0) & (pc < nc)), "column not in panel"
-                S = L[sb + np.arange(k)[None, :] * sld + np.arange(sld)[:, None]] if False else None
+                assert np.array_equal(trow[pr], Ls[roff + ra:roff + rb]), "row mask"
+                assert np.array_equal(int(pn["c0"]) + pc, Ls[roff + ca:roff + cb]), "column mask"
+                # every buffer offset of a loaded element stays inside the
+                # source's k columns (row < ld), and those inside L
+                assert max(rb, cb) <= sld
+                _chk(sb, sb + (k - 1) * sld + max(rb, cb), xsize, "task src")
                 A = L[sb + np.arange(k)[None, :] * sld + np.arange(ra, rb)[:, None]]
                 B = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
-                C = A @ B.T
-                mask = trow[pr][:, None] >= (int(pn["c0"]) + pc)[None, :]
-                sub = acc[np.ix_(pr, pc)]
-                acc[np.ix_(pr, pc)] = np.where(mask, sub - C, sub)
+                acc[np.ix_(pr, pc)] -= A @ B.T
             keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["r0"] == 0 else \
                 np.ones((nr, nc), bool)
             L[idx[keep]] = acc[keep]
