@@ -111,7 +111,9 @@ __device__ inline unsigned landing_off(uint64_t mask, int p, int base, int es) {
 // task lands rows in block row bi and columns in block column bj (uniform over
 // the wave) and, on the diagonal tile, bi >= bj (blocks above the diagonal are
 // never stored).  Operands come through buffer loads over the chunk's columns,
-// so rows that land nowhere and columns past k read as zero without branches.
+// so rows that land nowhere (all rows of an inactive block) and columns past k
+// read as zero without branches: a branch around a load would make the
+// compiler wait for it at the join instead of overlapping all of them.
 // With f32 the operands come from the FP32 copy of L (rounded to FP32) and the
 // products are accumulated in FP64 on the same MFMA.
 __device__ inline void task_update(const void* __restrict__ src, bool f32, int ld, int k, int ra,
@@ -134,20 +136,29 @@ __device__ inline void task_update(const void* __restrict__ src, bool f32, int l
                                                 (int64_t)(k - kb) * kstep);
     double a[4][4], bv[4][4];
     if (f32) {
+      // all loads first, conversions after, so that the loads overlap
+      float af[4][4], bf[4][4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          a[b][u] = ab[b] ? (double)load_f32(rs, ao[b] + 4 * u * kstep) : 0.0;
-          bv[b][u] = bb[b] ? (double)load_f32(rs, bo[b] + 4 * u * kstep) : 0.0;
+          af[b][u] = load_f32(rs, ao[b] + 4 * u * kstep);
+          bf[b][u] = load_f32(rs, bo[b] + 4 * u * kstep);
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          a[b][u] = (double)af[b][u];
+          bv[b][u] = (double)bf[b][u];
         }
     } else {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          a[b][u] = ab[b] ? load_f64(rs, ao[b] + 4 * u * kstep) : 0.0;
-          bv[b][u] = bb[b] ? load_f64(rs, bo[b] + 4 * u * kstep) : 0.0;
+          a[b][u] = load_f64(rs, ao[b] + 4 * u * kstep);
+          bv[b][u] = load_f64(rs, bo[b] + 4 * u * kstep);
         }
     }
 #pragma unroll
@@ -186,7 +197,9 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
   __shared__ double red[2][swc::PW * kLdsLd];
   const Tile tl = tiles[tile0 + blockIdx.x];
   const Panel pn = panels[tl.panel];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform over the wave: say so, so that the per-chunk
+  // buffer resources stay in scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nr = tl.nr, nc = pn.ncol;
   const bool diag = tl.r0 == 0;
   double* gtile = L + pn.vbase + tl.r0;
@@ -334,7 +347,7 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
   __shared__ double sx[64 * kLdsLd];  // solved tile columns, same layout
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nr = tl.nr, nc = pn.ncol;
   const int c0 = 16 * w;  // first column of this wave
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
