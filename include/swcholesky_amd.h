@@ -158,14 +158,15 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[11] = {panels, tiles, tasks, levels,
- * solve gathers, xsize, factor-list tiles, solve row blocks, contribution-vector
- * size, most row blocks in a level, diagonal-block scratch doubles}; swc_plan_export copies array `which` (0 panels,
- * 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->TRSM-list offsets, 5 panels
- * by level, 6 level->panel offsets, 7 solve gathers, 8 panel->gather offsets,
- * 9 factor-list tiles, 10 solve row blocks, 11 panel(by level)->row-block offsets,
- * 12 level->row-block offsets) as the packed structs of
- * swcholesky_amd/csrc/plan.h. */
+ * device, for tools and tests.  sizes[11] = {panels, tiles, tasks, levels, n,
+ * xsize, factor-list tiles, forward solve blocks, contribution-vector size, 0,
+ * diagonal-block scratch doubles}; swc_plan_export copies array `which` (0
+ * panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->factor-list
+ * offsets, 5 panels by level, 6 level->panel offsets, 7 column->gather offsets
+ * (int64, n+1), 8 gathered contribution offsets (int64, vsize), 9 factor-list
+ * tiles, 10 forward solve blocks, 11 level->solve-block offsets, 12 gathered
+ * columns by level, 13 level->gathered-column offsets) as the packed structs
+ * and arrays of swcholesky_amd/csrc/plan.h. */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
@@ -173,7 +174,7 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
 void swc_plan_free(swc_plan* p);
 int swc_plan_sizes(const swc_plan* p, int64_t* sizes, double* flops);
 int swc_plan_export(const swc_plan* p, int which, void* dst);
-/* Byte sizes of the exported structs {Panel, Tile, Task, SolveTask, SolveBlock}. */
+/* Byte sizes of the exported structs {Panel, Tile, Task, SolveBlock}. */
 int swc_plan_struct_sizes(int64_t* out);
 
 #ifdef __cplusplus

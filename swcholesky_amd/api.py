@@ -247,18 +247,16 @@ TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
                     ("ca", np.int32), ("cb", np.int32), ("kind", np.int32),
                     ("rmask", np.uint64), ("cmask", np.uint64)])
-STASK_DT = np.dtype([("voff", np.int64), ("rows_off", np.int32), ("ca", np.int32),
-                     ("cb", np.int32), ("wd", np.int32)])
-SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32), ("pad", np.int32)])
+SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
+                    ("writer", np.int32)])
 
 
 def build_plan(bcsc: BCSC):
     """The device work plan for a structure, as numpy record arrays (host only)."""
     lib = _lib.lib()
-    sizes = np.zeros(5, dtype=np.int64)
+    sizes = np.zeros(4, dtype=np.int64)
     check(lib.swc_plan_struct_sizes(ptr(sizes)))
-    want = [PANEL_DT.itemsize, TILE_DT.itemsize, TASK_DT.itemsize, STASK_DT.itemsize,
-            SBLK_DT.itemsize]
+    want = [PANEL_DT.itemsize, TILE_DT.itemsize, TASK_DT.itemsize, SBLK_DT.itemsize]
     if list(sizes) != want:
         raise RuntimeError(f"plan struct layout mismatch: library {list(sizes)}, python {want}")
     h = ctypes.c_void_p()
@@ -271,23 +269,23 @@ def build_plan(bcsc: BCSC):
         sz = np.zeros(11, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, nstask, xsize, ntrsm, nsblk, vsize, maxlb, dscr = (
+        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, _, dscr = (
             int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
-                (6, np.int32, nlevel + 1), (7, STASK_DT, nstask), (8, np.int32, npanel + 1),
-                (9, np.int32, ntrsm), (10, SBLK_DT, nsblk), (11, np.int32, npanel + 1),
-                (12, np.int32, nlevel + 1)]
+                (6, np.int32, nlevel + 1), (7, np.int64, n + 1), (8, np.int64, vsize),
+                (9, np.int32, ntrsm), (10, SBLK_DT, nfblk), (11, np.int32, nlevel + 1),
+                (12, np.int32, n), (13, np.int32, nlevel + 1)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
-                 "lvl_panels", "stasks", "stask_ptr", "trsm_tiles", "sblocks", "sblk_ptr",
-                 "lvl_sblk"]
+                 "lvl_panels", "gptr", "gidx", "trsm_tiles", "fblocks", "lvl_fblk",
+                 "gcols", "lvl_gcol"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)
             check(lib.swc_plan_export(h, which, ptr(a)))
             out[name] = a
         out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1],
-                   vsize=vsize, max_lvl_sblk=maxlb, diag_scratch=dscr)
+                   vsize=vsize, diag_scratch=dscr)
         return out
     finally:
         lib.swc_plan_free(h)

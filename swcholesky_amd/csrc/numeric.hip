@@ -23,7 +23,6 @@
 
 using swc::Panel;
 using swc::Plan;
-using swc::SolveTask;
 using swc::Task;
 using swc::Tile;
 
@@ -436,144 +435,165 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
 
 // ---- triangular solves --------------------------------------------------
 // Forward L y = b (blockedLsolve, Triangular_BCSC.h:14-49, restated level by
-// level): per panel, gather the descendants' contributions V_d = L21(d) x_d,
-// solve the panel's diagonal block, then (row-parallel) push L21 x into the
-// later columns of the same supernode (in place in b) and into V_s.
-// Backward L^T x = y: per panel, row-block partial sums of L21^T x, then a
-// fixed-order reduction and the transposed diagonal solve.
+// level).  The reference pushes L21(s) x_s into the rows below each supernode
+// s; here every supernode s keeps that product as its contribution vector V_s,
+// and column c takes the contributions of all its descendants in increasing
+// supernode order (the order of the pushes) when its supernode starts.  Each
+// panel then solves its diagonal block; its row blocks push L21 x into the
+// supernode's later columns (in place) or into V_s.  The right-hand sides are
+// accumulated in X and the panel solutions written to Y, so that no launch
+// reads what it writes.  Backward L^T x = y: per panel, L21^T x over the rows
+// below (ancestors, already final in X) and the transposed diagonal solve,
+// written to X.
 
 // Loads the panel's nc x nc diagonal block into LDS (column j at dl[j*kLdsLd]).
 __device__ inline void load_diag(const double* __restrict__ L, const Panel& pn, double* dl) {
-  for (int e = threadIdx.x; e < pn.ncol * pn.ncol; e += kThreads) {
+  for (int e = threadIdx.x; e < pn.ncol * pn.ncol; e += blockDim.x) {
     const int i = e % pn.ncol, j = e / pn.ncol;
     dl[j * kLdsLd + i] = L[pn.vbase + (int64_t)j * pn.ld + i];
   }
 }
 
+// x[c] -= V[gidx[e]] for e in [gptr[c], gptr[c+1]), one thread per gathered column.
 __global__ void __launch_bounds__(kThreads)
-k_fwd_diag(const double* __restrict__ L, const int* __restrict__ Ls,
-           const Panel* __restrict__ panels, const int* __restrict__ order, int p0,
-           const SolveTask* __restrict__ st, const int* __restrict__ stp,
-           const double* __restrict__ V, int64_t vld, double* __restrict__ X, int nrhs,
-           int64_t ldx) {
+k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __restrict__ gptr,
+             const int64_t* __restrict__ gidx, const double* __restrict__ V, int64_t vld,
+             double* __restrict__ X, int nrhs, int64_t ldx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ng) return;
+  const int c = gcols[g0 + t];
+  const int64_t e0 = gptr[c], e1 = gptr[c + 1];
+  for (int r = 0; r < nrhs; ++r) {
+    const double* v = V + (int64_t)r * vld;
+    double acc = X[(int64_t)r * ldx + c];
+#pragma unroll 8
+    for (int64_t e = e0; e < e1; ++e) acc -= v[gidx[e]];
+    X[(int64_t)r * ldx + c] = acc;
+  }
+}
+
+// One forward row block: solve the panel's diagonal block (every block of the
+// panel, from the same inputs), write the solution if this is the panel's
+// writer block, then push L21 x of its rows.
+__global__ void __launch_bounds__(kThreads)
+k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ panels,
+            const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
+            int64_t vld, double* __restrict__ X, double* __restrict__ Y, int nrhs, int64_t ldx,
+            int64_t ldy) {
   __shared__ double dl[swc::PW * kLdsLd];
-  __shared__ double part[kThreads / 64][swc::PW];
-  const int q = order[p0 + blockIdx.x];
-  const Panel pn = panels[q];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double xs[swc::PW];
+  __shared__ double part[kThreads / 64][swc::TM];
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
+  const Panel pn = panels[bk.panel];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = pn.ncol;
   load_diag(L, pn, dl);
+  __syncthreads();
+  const double* blk = L + pn.vbase + bk.r0;
   for (int r = 0; r < nrhs; ++r) {
     double* x = X + (int64_t)r * ldx;
-    const double* v = V + (int64_t)r * vld;
-    part[wave][lane] = 0.0;
-    __syncthreads();
-    // wave w gathers sources w, w+4, ...; rows of one source are distinct
-    for (int s = stp[q] + wave; s < stp[q + 1]; s += kThreads / 64) {
-      const SolveTask tk = st[s];
-      for (int i = tk.ca + lane; i < tk.cb; i += 64)
-        part[wave][Ls[tk.rows_off + i] - pn.c0] += v[tk.voff + i - tk.wd];
-    }
-    __syncthreads();
+    double* v = V + (int64_t)r * vld;
     if (wave == 0) {
-      double ti = 0.0;
-      if (lane < nc) ti = x[pn.c0 + lane] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+      double ti = lane < nc ? x[pn.c0 + lane] : 0.0;
       for (int j = 0; j < nc; ++j) {
         const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
         if (lane == j) ti = xj;
         else if (lane > j) ti -= dl[j * kLdsLd + lane] * xj;
       }
-      if (lane < nc) x[pn.c0 + lane] = ti;
+      if (lane < nc) {
+        xs[lane] = ti;
+        if (bk.writer) Y[(int64_t)r * ldy + pn.c0 + lane] = ti;
+      }
+    }
+    __syncthreads();
+    if (bk.nr > 0) {
+      // wave w sums columns j = w, w+4, ... for row `lane`
+      double acc = 0.0;
+      if (lane < bk.nr)
+        for (int j = wave; j < nc; j += kThreads / 64) acc = fma(blk[(int64_t)j * pn.ld + lane], xs[j], acc);
+      part[wave][lane] = acc;
+      __syncthreads();
+      if (wave == 0 && lane < bk.nr) {
+        const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+        const int p = (pn.c0 - pn.sc0) + bk.r0 + lane;  // position in the supernode's rows
+        if (p < pn.sw) x[pn.sc0 + p] -= y;               // a later column of the supernode
+        else v[pn.voff + (p - pn.sw)] += y;              // a row of an ancestor
+      }
     }
     __syncthreads();
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_fwd_update(const double* __restrict__ L, const Panel* __restrict__ panels,
-             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
-             int64_t vld, double* __restrict__ X, int nrhs, int64_t ldx) {
-  __shared__ double xs[swc::PW];
-  __shared__ double part[kThreads / 64][swc::TM];
-  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
-  const Panel pn = panels[bk.panel];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nc = pn.ncol;
-  const double* blk = L + pn.vbase + bk.r0;
-  for (int r = 0; r < nrhs; ++r) {
-    double* x = X + (int64_t)r * ldx;
-    double* v = V + (int64_t)r * vld;
-    if (tid < nc) xs[tid] = x[pn.c0 + tid];
-    __syncthreads();
-    // wave w sums columns j = w, w+4, ... for row `lane`
-    double acc = 0.0;
-    if (lane < bk.nr)
-      for (int j = wave; j < nc; j += kThreads / 64) acc = fma(blk[(int64_t)j * pn.ld + lane], xs[j], acc);
-    part[wave][lane] = acc;
-    __syncthreads();
-    if (wave == 0 && lane < bk.nr) {
-      const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-      const int p = (pn.c0 - pn.sc0) + bk.r0 + lane;  // position in the supernode's rows
-      if (p < pn.sw) x[pn.sc0 + p] -= y;               // a later column of the supernode
-      else v[pn.voff + (p - pn.sw)] += y;              // a row of an ancestor
-    }
-    __syncthreads();
-  }
-}
-
+// Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
+// for the panel's columns j, into Pb at the block's slot in its level.  Wave w
+// owns columns j = w + 4q, lane l row l of the block; each column's lane
+// products are combined by a fixed butterfly (deterministic).
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ panels, const swc::SolveBlock* __restrict__ blocks,
               int b0, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
-  __shared__ double xs[swc::TM];
-  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
   const Panel pn = panels[bk.panel];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = pn.ncol;
   const double* blk = L + pn.vbase + bk.r0;
+  const int row = lane < bk.nr ? Ls[pn.rows_off + bk.r0 + lane] : 0;
   for (int r = 0; r < nrhs; ++r) {
-    const double* x = X + (int64_t)r * ldx;
+    const double xr = lane < bk.nr ? X[(int64_t)r * ldx + row] : 0.0;
     double* out = Pb + (int64_t)r * pld + (int64_t)blockIdx.x * swc::PW;
-    if (tid < bk.nr) xs[tid] = x[Ls[pn.rows_off + bk.r0 + tid]];
-    __syncthreads();
-    for (int j = wave; j < nc; j += kThreads / 64) {
-      double acc = lane < bk.nr ? blk[(int64_t)j * pn.ld + lane] * xs[lane] : 0.0;
-      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-      if (lane == 0) out[j] = acc;
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = wave + 4 * q;
+      a[q] = (j < nc && lane < bk.nr) ? blk[(int64_t)j * pn.ld + lane] * xr : 0.0;
     }
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
+      const int j = wave + 4 * q;
+      if (lane == 0 && j < nc) out[j] = a[q];
+    }
   }
 }
 
+// Backward panel: t = y_p - (sum of its row blocks' partials), x_p = L11^{-T} t.
+// The partials are summed by wave w over blocks w, w+4, ... and the four sums
+// combined in a fixed order.
 __global__ void __launch_bounds__(kThreads)
 k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ panels,
-           const int* __restrict__ order, int p0, const int* __restrict__ sblk_ptr, int b0,
-           const double* __restrict__ Pb, int64_t pld, double* __restrict__ X, int nrhs,
-           int64_t ldx) {
+           const int* __restrict__ order, int p0, const int* __restrict__ fblk_ptr, int b0,
+           const double* __restrict__ Pb, int64_t pld, const double* __restrict__ Y,
+           double* __restrict__ X, int nrhs, int64_t ldx, int64_t ldy) {
   __shared__ double dl[swc::PW * kLdsLd];
+  __shared__ double part[kThreads / 64][swc::PW];
   const int qi = p0 + blockIdx.x;
   const Panel pn = panels[order[qi]];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = pn.ncol;
+  const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   load_diag(L, pn, dl);
-  __syncthreads();
-  if (wave != 0) return;
   for (int r = 0; r < nrhs; ++r) {
-    double* x = X + (int64_t)r * ldx;
     const double* pb = Pb + (int64_t)r * pld;
-    double ti = 0.0;
-    if (lane < nc) {
-      double sum = 0.0;
-      for (int b = sblk_ptr[qi]; b < sblk_ptr[qi + 1]; ++b) sum += pb[(int64_t)(b - b0) * swc::PW + lane];
-      ti = x[pn.c0 + lane] - sum;
+    double sum = 0.0;
+    if (lane < nc)
+      for (int b = f0 + wave; b < f1; b += kThreads / 64) sum += pb[(int64_t)(b - b0) * swc::PW + lane];
+    part[wave][lane] = sum;
+    __syncthreads();
+    if (wave == 0) {
+      double ti = 0.0;
+      if (lane < nc)
+        ti = Y[(int64_t)r * ldy + pn.c0 + lane] -
+             (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+      for (int j = nc - 1; j >= 0; --j) {
+        const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
+        if (lane == j) ti = xj;
+        else if (lane < j) ti -= dl[lane * kLdsLd + j] * xj;
+      }
+      if (lane < nc) X[(int64_t)r * ldx + pn.c0 + lane] = ti;
     }
-    for (int j = nc - 1; j >= 0; --j) {
-      const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
-      if (lane == j) ti = xj;
-      else if (lane < j) ti -= dl[lane * kLdsLd + j] * xj;
-    }
-    if (lane < nc) x[pn.c0 + lane] = ti;
+    __syncthreads();
   }
 }
 
@@ -864,12 +884,14 @@ struct swc_handle {
   Panel* dPanels = nullptr;
   Tile* dTiles = nullptr;
   Task* dTasks = nullptr;
-  SolveTask* dSt = nullptr;
-  int* dStp = nullptr;
-  swc::SolveBlock* dSblk = nullptr;
-  int* dSblkPtr = nullptr;
+  int64_t* dGptr = nullptr;  // forward gathers: per column, offsets into dGidx
+  int64_t* dGidx = nullptr;
+  int* dGcols = nullptr;     // gathered columns by level
+  swc::SolveBlock* dFblk = nullptr;
+  int* dFblkPtr = nullptr;  // per panel (level order): its forward/backward row blocks
   double* dV = nullptr;   // contribution vectors, vsize * work_cols
-  double* dPb = nullptr;  // backward partial sums, max_lvl_sblk * PW * work_cols
+  double* dPb = nullptr;  // backward partial sums, max_lvl_fblk * PW * work_cols
+  double* dY = nullptr;   // forward solutions, n * work_cols
   int* dOrder = nullptr;
   int* dTrsm = nullptr;
   double* dDiag = nullptr;  // per-level copies of the diagonal blocks being factored
@@ -937,14 +959,14 @@ struct swc_handle {
   void release() {
     drop_graphs();
     hipFree(dL); hipFree(dLf); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
-    hipFree(dSt); hipFree(dStp); hipFree(dSblk); hipFree(dSblkPtr); hipFree(dV); hipFree(dPb); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
+    hipFree(dGptr); hipFree(dGidx); hipFree(dGcols); hipFree(dFblk); hipFree(dFblkPtr); hipFree(dV); hipFree(dPb); hipFree(dY); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
     hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart); hipFree(dNormPart);
     hipFree(dRef);
     hipFree(dZ); hipFree(dArena); hipFree(dNodes); hipFree(dCol2sup); hipFree(dSpx);
     hipFree(dSsuper); hipFree(dSpi); hipFree(dDiagPos); hipFree(dQi); hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dSt = nullptr; dStp = nullptr; dSblk = nullptr; dSblkPtr = nullptr; dV = nullptr; dPb = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     dRef = nullptr;
@@ -1011,10 +1033,11 @@ int ensure_work(swc_handle* h, int cols) {
   h->sgraphs.clear();
   hipFree(h->dWork);
   hipFree(h->dV);
+  hipFree(h->dY);
   hipFree(h->dPb);
   hipFree(h->dPart);
   hipFree(h->dRef);
-  h->dWork = h->dV = h->dPb = h->dPart = h->dRef = nullptr;
+  h->dWork = h->dV = h->dY = h->dPb = h->dPart = h->dRef = nullptr;
   h->work_cols = 0;
   const Plan& P = h->plan;
   SWC_HIP(hipMalloc((void**)&h->dPart, sizeof(double) * (size_t)kRedBlocks * cols));
@@ -1024,7 +1047,8 @@ int ensure_work(swc_handle* h, int cols) {
   SWC_HIP(hipMalloc((void**)&h->dRef, sizeof(double) * (size_t)5 * h->n * cols));
   SWC_HIP(hipMalloc((void**)&h->dWork, sizeof(double) * (size_t)h->n * cols));
   SWC_HIP(hipMalloc((void**)&h->dV, sizeof(double) * (size_t)std::max<int64_t>(P.vsize, 1) * cols));
-  SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_sblk, 1) * swc::PW * cols));
+  SWC_HIP(hipMalloc((void**)&h->dY, sizeof(double) * (size_t)h->n * cols));
+  SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_fblk, 1) * swc::PW * cols));
   h->work_cols = cols;
   return SWC_OK;
 }
@@ -1145,33 +1169,34 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
-  const int64_t pld = (int64_t)std::max(P.max_lvl_sblk, 1) * swc::PW;
+  const int64_t ldy = h->n;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
   for (int l = 0; l < P.nlevel; ++l) {
-    const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
-    const int b0 = P.lvl_sblk[l], nb = P.lvl_sblk[l + 1] - b0;
-    if (np > 0)
-      hipLaunchKernelGGL(k_fwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
-                         (const SolveTask*)h->dSt, (const int*)h->dStp, (const double*)h->dV, vld,
-                         dX, nrhs, ldx);
+    const int g0 = P.lvl_gcol[l], ng = P.lvl_gcol[l + 1] - g0;
+    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
+    if (ng > 0)
+      hipLaunchKernelGGL(k_fwd_gather, dim3((ng + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                         (const int*)h->dGcols, g0, ng, (const int64_t*)h->dGptr,
+                         (const int64_t*)h->dGidx, (const double*)h->dV, vld, dX, nrhs, ldx);
     if (nb > 0)
-      hipLaunchKernelGGL(k_fwd_update, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const Panel*)h->dPanels, (const swc::SolveBlock*)h->dSblk, b0, h->dV,
-                         vld, dX, nrhs, ldx);
+      hipLaunchKernelGGL(k_fwd_panel, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
+                         (const Panel*)h->dPanels, (const swc::SolveBlock*)h->dFblk, b0, h->dV,
+                         vld, dX, h->dY, nrhs, ldx, ldy);
   }
+  const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::PW;
   for (int l = P.nlevel - 1; l >= 0; --l) {
     const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
-    const int b0 = P.lvl_sblk[l], nb = P.lvl_sblk[l + 1] - b0;
+    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
     if (nb > 0)
       hipLaunchKernelGGL(k_bwd_partial, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, (const Panel*)h->dPanels,
-                         (const swc::SolveBlock*)h->dSblk, b0, h->dPb, pld, (const double*)dX,
+                         (const swc::SolveBlock*)h->dFblk, b0, h->dPb, pld, (const double*)dX,
                          nrhs, ldx);
     if (np > 0)
       hipLaunchKernelGGL(k_bwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
-                         (const int*)h->dSblkPtr, b0, (const double*)h->dPb, pld, dX, nrhs, ldx);
+                         (const int*)h->dFblkPtr, b0, (const double*)h->dPb, pld,
+                         (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
 }
 
@@ -1553,10 +1578,11 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dPanels, P.panels.data(), P.panels.size(), st);
   if (!rc) rc = upload(&h->dTiles, P.tiles.data(), P.tiles.size(), st);
   if (!rc) rc = upload(&h->dTasks, P.tasks.data(), P.tasks.size(), st);
-  if (!rc) rc = upload(&h->dSt, P.stasks.data(), P.stasks.size(), st);
-  if (!rc) rc = upload(&h->dStp, P.stask_ptr.data(), P.stask_ptr.size(), st);
-  if (!rc) rc = upload(&h->dSblk, P.sblocks.data(), P.sblocks.size(), st);
-  if (!rc) rc = upload(&h->dSblkPtr, P.sblk_ptr.data(), P.sblk_ptr.size(), st);
+  if (!rc) rc = upload(&h->dGptr, P.gptr.data(), P.gptr.size(), st);
+  if (!rc) rc = upload(&h->dGidx, P.gidx.data(), P.gidx.size(), st);
+  if (!rc) rc = upload(&h->dGcols, P.gcols.data(), P.gcols.size(), st);
+  if (!rc) rc = upload(&h->dFblk, P.fblocks.data(), P.fblocks.size(), st);
+  if (!rc) rc = upload(&h->dFblkPtr, P.fblk_ptr.data(), P.fblk_ptr.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
   if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
   if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);

@@ -72,7 +72,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     }
   }
 
-  auto describe = [&](int q, const Src& sr, Task& t, SolveTask& st) {
+  auto describe = [&](int q, const Src& sr, Task& t) {
     const Panel& pn = P.panels[q];
     if (sr.d >= 0) {
       const int d = sr.d;
@@ -92,22 +92,25 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     t.ca = sr.ca;
     t.cb = sr.cb;
     t.kind = sr.d >= 0 ? 0 : 1;
-    (void)st;
   };
 
-  // ---- forward-solve gathers: descendants touching each panel ----
-  P.stask_ptr.assign(P.npanel + 1, 0);
-  for (int q = 0; q < P.npanel; ++q) {
-    for (const Src& sr : srcs[q]) {
-      SolveTask st;
-      st.voff = voff[sr.d];
-      st.rows_off = (int)pi[sr.d];
-      st.ca = sr.ca;
-      st.cb = sr.cb;
-      st.wd = super[sr.d + 1] - super[sr.d];
-      P.stasks.push_back(st);
+  // ---- forward-solve gathers: column c takes row c of every descendant's
+  // contribution vector, in increasing supernode order ----
+  P.gptr.assign((size_t)n + 1, 0);
+  for (int d = 0; d < nsuper; ++d) {
+    const int W = super[d + 1] - super[d];
+    for (int64_t i = pi[d] + W; i < pi[d + 1]; ++i) P.gptr[(size_t)Ls[i] + 1]++;
+  }
+  for (int c = 0; c < n; ++c) P.gptr[c + 1] += P.gptr[c];
+  if (P.gptr[n] != P.vsize) throw std::logic_error("gather lists do not cover V");
+  P.gidx.resize((size_t)P.vsize);
+  {
+    std::vector<int64_t> fill(P.gptr.begin(), P.gptr.end() - 1);
+    for (int d = 0; d < nsuper; ++d) {
+      const int W = super[d + 1] - super[d];
+      for (int64_t i = pi[d] + W; i < pi[d + 1]; ++i)
+        P.gidx[(size_t)fill[Ls[i]]++] = voff[d] + (i - pi[d] - W);
     }
-    P.stask_ptr[q + 1] = (int)P.stasks.size();
   }
 
   // ---- levels: tiles of each level form one update launch, followed by one
@@ -142,8 +145,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     const int vlast = Ls[pn.rows_off + tl.r0 + tl.nr - 1];
     for (const Src& sr : srcs[q]) {
       Task t;
-      SolveTask st;
-      describe(q, sr, t, st);
+      describe(q, sr, t);
       const int* rows = Ls + t.rows_off;
       const int len = (int)(pi[sr.d + 1] - pi[sr.d]);
       const int ra = std::max(lower_idx(rows, len, vfirst), sr.ca);
@@ -161,8 +163,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     const Panel& pn = P.panels[q];
     const int off = pn.c0 - super[pn.sn];
     Task t;
-    SolveTask st;
-    describe(q, {-(q2 + 1), off, off + pn.ncol}, t, st);
+    describe(q, {-(q2 + 1), off, off + pn.ncol}, t);
     t.ra = off + tl.r0;
     t.rb = t.ra + tl.nr;
     ptasks[ti].push_back(t);
@@ -212,26 +213,52 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
   }
 
-  // ---- solve row blocks below each panel's diagonal block, panels by level ----
-  P.sblk_ptr.assign(P.npanel + 1, 0);
-  P.lvl_sblk.assign(P.nlevel + 1, 0);
+  // ---- forward solve blocks and gathered columns, by level ----
+  P.lvl_fblk.assign(P.nlevel + 1, 0);
+  P.fblk_ptr.assign(P.npanel + 1, 0);
   for (int l = 0; l < P.nlevel; ++l) {
     for (int qi = P.lvl_panels[l]; qi < P.lvl_panels[l + 1]; ++qi) {
-      const Panel& pn = P.panels[P.panel_order[qi]];
-      P.sblk_ptr[qi] = (int)P.sblocks.size();
-      for (int r0 = pn.ncol; r0 < pn.nrow; r0 += TM)
-        P.sblocks.push_back({P.panel_order[qi], r0, std::min(TM, pn.nrow - r0), 0});
+      const int q = P.panel_order[qi];
+      const Panel& pn = P.panels[q];
+      P.fblk_ptr[qi] = (int)P.fblocks.size();
+      if (pn.nrow == pn.ncol) {
+        P.fblocks.push_back({q, pn.ncol, 0, 1});
+      } else {
+        for (int r0 = pn.ncol; r0 < pn.nrow; r0 += TM)
+          P.fblocks.push_back({q, r0, std::min(TM, pn.nrow - r0), r0 == pn.ncol ? 1 : 0});
+      }
     }
-    P.lvl_sblk[l + 1] = (int)P.sblocks.size();
-    P.max_lvl_sblk = std::max(P.max_lvl_sblk, P.lvl_sblk[l + 1] - P.lvl_sblk[l]);
+    P.lvl_fblk[l + 1] = (int)P.fblocks.size();
+    P.max_lvl_fblk = std::max(P.max_lvl_fblk, P.lvl_fblk[l + 1] - P.lvl_fblk[l]);
   }
-  P.sblk_ptr[P.npanel] = (int)P.sblocks.size();
+  P.fblk_ptr[P.npanel] = (int)P.fblocks.size();
+  P.lvl_gcol.assign(P.nlevel + 1, 0);
+  for (int s = 0; s < nsuper; ++s)
+    P.lvl_gcol[P.panel_level[first_panel[s]] + 1] += super[s + 1] - super[s];
+  for (int l = 0; l < P.nlevel; ++l) P.lvl_gcol[l + 1] += P.lvl_gcol[l];
+  P.gcols.resize(n);
+  {
+    std::vector<int> fill(P.lvl_gcol.begin(), P.lvl_gcol.end() - 1);
+    for (int s = 0; s < nsuper; ++s) {
+      int& f = fill[P.panel_level[first_panel[s]]];
+      for (int c = super[s]; c < super[s + 1]; ++c) P.gcols[f++] = c;
+    }
+  }
 
   // ---- flatten: tiles by level (stable), tasks, TRSM lists ----
+  // Within a level the most expensive tiles go first (longest-processing-time
+  // order), so that the level does not end on a long tile started last.  The
+  // estimate is the operand volume plus a fixed latency per task.
+  std::vector<double> cost(ptiles.size(), 0.0);
+  for (size_t i = 0; i < ptiles.size(); ++i)
+    for (const Task& t : ptasks[i])
+      cost[i] += (double)t.k * ((t.rb - t.ra) + (t.cb - t.ca)) + 2048.0;
   std::vector<int> order(ptiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return ptiles[a].level < ptiles[b].level; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (ptiles[a].level != ptiles[b].level) return ptiles[a].level < ptiles[b].level;
+    return cost[a] > cost[b];
+  });
   P.lvl_tiles.assign(P.nlevel + 1, 0);
   P.lvl_trsm.assign(P.nlevel + 1, 0);
   std::vector<std::vector<int>> trsm_by_level(P.nlevel);
@@ -364,8 +391,7 @@ int swc_plan_struct_sizes(int64_t* out) {
   out[0] = sizeof(swc::Panel);
   out[1] = sizeof(swc::Tile);
   out[2] = sizeof(swc::Task);
-  out[3] = sizeof(swc::SolveTask);
-  out[4] = sizeof(swc::SolveBlock);
+  out[3] = sizeof(swc::SolveBlock);
   return SWC_OK;
 }
 
@@ -376,12 +402,12 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[1] = (int64_t)P.tiles.size();
   out[2] = (int64_t)P.tasks.size();
   out[3] = P.nlevel;
-  out[4] = (int64_t)P.stasks.size();
+  out[4] = P.n;
   out[5] = P.xsize;
   out[6] = (int64_t)P.trsm_tiles.size();
-  out[7] = (int64_t)P.sblocks.size();
+  out[7] = (int64_t)P.fblocks.size();
   out[8] = P.vsize;
-  out[9] = P.max_lvl_sblk;
+  out[9] = 0;
   out[10] = P.diag_scratch;
   if (flops) {
     flops[0] = P.update_flops;
@@ -405,12 +431,13 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 4: return cp(P.lvl_trsm);
     case 5: return cp(P.panel_order);
     case 6: return cp(P.lvl_panels);
-    case 7: return cp(P.stasks);
-    case 8: return cp(P.stask_ptr);
+    case 7: return cp(P.gptr);
+    case 8: return cp(P.gidx);
     case 9: return cp(P.trsm_tiles);
-    case 10: return cp(P.sblocks);
-    case 11: return cp(P.sblk_ptr);
-    case 12: return cp(P.lvl_sblk);
+    case 10: return cp(P.fblocks);
+    case 11: return cp(P.lvl_fblk);
+    case 12: return cp(P.gcols);
+    case 13: return cp(P.lvl_gcol);
     default: return swc_set_error(SWC_EINVAL, "swc_plan_export: unknown array id");
   }
 }

@@ -62,28 +62,19 @@ struct Task {
   uint64_t cmask;    // panel columns receiving source rows ca..cb-1
 };
 
-// Forward-solve gather into one panel from a descendant supernode d: rows
-// [ca,cb) of d's row list lie in the panel's columns; their values are taken
-// from d's contribution vector V_d = L21(d) x_d at V[voff + i - wd].
-struct SolveTask {
-  int64_t voff;
-  int rows_off;
-  int ca, cb;
-  int wd;
-};
-
-// Rows [r0, r0+nr) of a panel below its diagonal block (solve mat-vecs).
+// Forward-solve row block of a panel: rows [r0, r0+nr) below its diagonal
+// block (nr = 0: the panel has no rows below it).  The block with writer = 1
+// also stores the panel's solution.
 struct SolveBlock {
   int panel;
   int r0;
   int nr;
-  int pad;
+  int writer;
 };
 
 static_assert(sizeof(Panel) == 48, "Panel layout is part of the plan export");
 static_assert(sizeof(Tile) == 32, "Tile layout is part of the plan export");
 static_assert(sizeof(Task) == 56, "Task layout is part of the plan export");
-static_assert(sizeof(SolveTask) == 24, "SolveTask layout is part of the plan export");
 static_assert(sizeof(SolveBlock) == 16, "SolveBlock layout is part of the plan export");
 
 struct Plan {
@@ -100,12 +91,18 @@ struct Plan {
   std::vector<int> panel_level;     // per panel
   std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order
   std::vector<int> panel_order;     // panels grouped by level
-  std::vector<SolveTask> stasks;    // per panel, descendant gathers of the forward solve
-  std::vector<int> stask_ptr;       // npanel+1
-  std::vector<SolveBlock> sblocks;  // row blocks below each panel's diagonal, by level
-  std::vector<int> sblk_ptr;        // npanel+1 into sblocks (panels in panel_order)
-  std::vector<int> lvl_sblk;        // nlevel+1 into sblocks
-  int max_lvl_sblk = 0;             // most row blocks in one level
+  // forward solve: column c (factor ordering) takes the contributions
+  // V[gidx[gptr[c] .. gptr[c+1])] of its descendants, in increasing supernode
+  // order; the columns of the supernodes whose first panel is at level l are
+  // gathered at that level (gcols[lvl_gcol[l] ..])
+  std::vector<int64_t> gptr;        // n+1
+  std::vector<int64_t> gidx;        // vsize
+  std::vector<int> gcols;           // n
+  std::vector<int> lvl_gcol;        // nlevel+1
+  std::vector<SolveBlock> fblocks;  // row blocks of each level's panels (both sweeps)
+  std::vector<int> lvl_fblk;        // nlevel+1
+  std::vector<int> fblk_ptr;        // npanel+1: blocks of panel_order[i]
+  int max_lvl_fblk = 0;             // most row blocks in one level
   int64_t diag_scratch = 0;         // doubles of diagonal-block copies one level needs
   int64_t vsize = 0;                // sum over supernodes of (rows - width)
   double update_flops = 0, factor_flops = 0;

@@ -96,14 +96,20 @@ def factor(plan, Ls, xsize, amap, vals):
 
 def solve(plan, Ls, L, b):
     """Forward then backward solve in the factor ordering (b is an n-vector),
-    as k_fwd_diag / k_fwd_update / k_bwd_partial / k_bwd_diag run it."""
+    as k_fwd_gather / k_fwd_panel / k_bwd_panel run it: the forward sweep
+    accumulates right-hand sides in x and writes the panel solutions to y; the
+    backward sweep reads y and the final values of x, and overwrites x."""
     x = np.array(b, dtype=np.float64)
     n = len(x)
-    panels, st, stp, order = plan["panels"], plan["stasks"], plan["stask_ptr"], plan["panel_order"]
-    lp, blocks, lb, sbp = plan["lvl_panels"], plan["sblocks"], plan["lvl_sblk"], plan["sblk_ptr"]
+    y = np.full(n, np.nan)
+    panels, order, lp = plan["panels"], plan["panel_order"], plan["lvl_panels"]
+    gptr, gidx, gcols, lg = plan["gptr"], plan["gidx"], plan["gcols"], plan["lvl_gcol"]
+    fb, lf = plan["fblocks"], plan["lvl_fblk"]
     vsize = plan["vsize"]
     V = np.zeros(max(vsize, 1))
     xsize = len(L)
+    assert len(gidx) == vsize and (vsize == 0 or (gidx.min() >= 0 and gidx.max() < vsize))
+    assert np.array_equal(np.sort(gcols), np.arange(n)), "every column gathered once"
 
     def diag(pn):
         nc, ld = int(pn["ncol"]), int(pn["ld"])
@@ -112,54 +118,38 @@ def solve(plan, Ls, L, b):
         return np.tril(L[idx])
 
     for lvl in range(plan["nlevel"]):
-        for qi in range(lp[lvl], lp[lvl + 1]):
-            q = order[qi]
-            pn = panels[q]
-            nc, c0 = int(pn["ncol"]), int(pn["c0"])
-            t = x[c0:c0 + nc].copy()
-            for s_ in range(stp[q], stp[q + 1]):
-                tk = st[s_]
-                ca, cb, ro, wd, vo = (int(tk[f]) for f in ("ca", "cb", "rows_off", "wd", "voff"))
-                assert ca >= wd
-                _chk(vo + ca - wd, vo + cb - wd, vsize, "V gather")
-                pos = Ls[ro + ca:ro + cb] - c0
-                assert np.all((pos >= 0) & (pos < nc))
-                t[pos] -= V[vo + np.arange(ca, cb) - wd]
-            x[c0:c0 + nc] = np.linalg.solve(diag(pn), t)
-        for bk in blocks[lb[lvl]:lb[lvl + 1]]:
+        for c in gcols[lg[lvl]:lg[lvl + 1]]:
+            for e in range(gptr[c], gptr[c + 1]):
+                x[c] -= V[gidx[e]]
+        pushes = []
+        for bk in fb[lf[lvl]:lf[lvl + 1]]:
             pn = panels[bk["panel"]]
             nc, ld, c0 = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"])
             r0, nr = int(bk["r0"]), int(bk["nr"])
-            assert 1 <= nr <= 64 and r0 >= nc and r0 + nr <= int(pn["nrow"])
-            blk = L[int(pn["vbase"]) + r0 + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]]
-            y = blk @ x[c0:c0 + nc]
-            p = (c0 - int(pn["sc0"])) + r0 + np.arange(nr)
+            assert 0 <= nr <= 64 and r0 >= nc and r0 + nr <= int(pn["nrow"])
+            xs = np.linalg.solve(diag(pn), x[c0:c0 + nc])
+            if bk["writer"]:
+                y[c0:c0 + nc] = xs
+            if nr:
+                blk = L[int(pn["vbase"]) + r0 + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]]
+                pushes.append((pn, r0, nr, blk @ xs))
+        for pn, r0, nr, yy in pushes:  # every read of x in the level precedes these writes
+            p = (int(pn["c0"]) - int(pn["sc0"])) + r0 + np.arange(nr)
             intra = p < int(pn["sw"])
             tgt = int(pn["sc0"]) + p[intra]
             _chk(int(tgt.min(initial=0)), int(tgt.max(initial=0)) + 1, n, "intra")
-            x[tgt] -= y[intra]
+            x[tgt] -= yy[intra]
             vi = int(pn["voff"]) + p[~intra] - int(pn["sw"])
             if vi.size:
                 _chk(int(vi.min()), int(vi.max()) + 1, vsize, "V push")
-            V[vi] += y[~intra]
+            V[vi] += yy[~intra]
+    assert not np.isnan(y).any(), "a panel solution was never written"
     for lvl in range(plan["nlevel"] - 1, -1, -1):
-        b0 = lb[lvl]
-        part = {}
-        for bi in range(lb[lvl], lb[lvl + 1]):
-            bk = blocks[bi]
-            pn = panels[bk["panel"]]
-            nc, ld = int(pn["ncol"]), int(pn["ld"])
-            r0, nr = int(bk["r0"]), int(bk["nr"])
-            assert bi - b0 < plan["max_lvl_sblk"]
-            blk = L[int(pn["vbase"]) + r0 + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]]
-            rows = Ls[int(pn["rows_off"]) + r0:int(pn["rows_off"]) + r0 + nr]
-            part[bi] = blk.T @ x[rows]
         for qi in range(lp[lvl], lp[lvl + 1]):
             pn = panels[order[qi]]
-            nc, c0 = int(pn["ncol"]), int(pn["c0"])
-            t = x[c0:c0 + nc].copy()
-            for bi in range(sbp[qi], sbp[qi + 1]):
-                assert lb[lvl] <= bi < lb[lvl + 1]
-                t -= part[bi]
+            nc, ld, c0, nrow = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"]), int(pn["nrow"])
+            rows = Ls[int(pn["rows_off"]) + nc:int(pn["rows_off"]) + nrow]
+            blk = L[int(pn["vbase"]) + nc + np.arange(nc)[None, :] * ld + np.arange(nrow - nc)[:, None]]
+            t = y[c0:c0 + nc] - blk.T @ x[rows]
             x[c0:c0 + nc] = np.linalg.solve(diag(pn).T, t)
     return x
