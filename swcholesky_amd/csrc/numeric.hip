@@ -480,25 +480,33 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ panels,
             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
             int64_t vld, double* __restrict__ X, double* __restrict__ Y, int nrhs, int64_t ldx,
             int64_t ldy) {
-  __shared__ double dl[swc::PW * kLdsLd];
   __shared__ double xs[swc::PW];
   __shared__ double part[kThreads / 64][swc::TM];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
   const Panel pn = panels[bk.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = pn.ncol;
-  load_diag(L, pn, dl);
-  __syncthreads();
+  // wave 0 holds row `lane` of L11 (zero beyond nc) and the reciprocal of its
+  // diagonal entry (1 beyond nc, which leaves the padded unknowns at zero)
+  double row[swc::PW];
+  double rinv = 1.0;
+  if (wave == 0) {
+#pragma unroll
+    for (int c = 0; c < swc::PW; ++c)
+      row[c] = (lane < nc && c < lane) ? L[pn.vbase + (int64_t)c * pn.ld + lane] : 0.0;
+    if (lane < nc) rinv = 1.0 / L[pn.vbase + (int64_t)lane * pn.ld + lane];
+  }
   const double* blk = L + pn.vbase + bk.r0;
   for (int r = 0; r < nrhs; ++r) {
     double* x = X + (int64_t)r * ldx;
     double* v = V + (int64_t)r * vld;
     if (wave == 0) {
       double ti = lane < nc ? x[pn.c0 + lane] : 0.0;
-      for (int j = 0; j < nc; ++j) {
-        const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
-        if (lane == j) ti = xj;
-        else if (lane > j) ti -= dl[j * kLdsLd + lane] * xj;
+      // column-oriented substitution: x_j = t_j / L_jj, then t_i -= L_ij x_j
+#pragma unroll
+      for (int j = 0; j < swc::PW; ++j) {
+        const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+        ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
       }
       if (lane < nc) {
         xs[lane] = ti;
@@ -574,6 +582,16 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ panels,
   const int nc = pn.ncol;
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   load_diag(L, pn, dl);
+  __syncthreads();
+  // wave 0 holds column `lane` of L11 below the diagonal (zero beyond nc) and
+  // the reciprocal of its diagonal entry
+  double col[swc::PW];
+  double rinv = 1.0;
+  if (wave == 0) {
+#pragma unroll
+    for (int j = 0; j < swc::PW; ++j) col[j] = (lane < nc && j > lane && j < nc) ? dl[lane * kLdsLd + j] : 0.0;
+    if (lane < nc) rinv = 1.0 / dl[lane * kLdsLd + lane];
+  }
   for (int r = 0; r < nrhs; ++r) {
     const double* pb = Pb + (int64_t)r * pld;
     double sum = 0.0;
@@ -586,10 +604,11 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ panels,
       if (lane < nc)
         ti = Y[(int64_t)r * ldy + pn.c0 + lane] -
              (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
-      for (int j = nc - 1; j >= 0; --j) {
-        const double xj = __shfl(ti, j) / dl[j * kLdsLd + j];
-        if (lane == j) ti = xj;
-        else if (lane < j) ti -= dl[lane * kLdsLd + j] * xj;
+      // x_j = t_j / L_jj, then t_i -= L_ji x_j for i < j
+#pragma unroll
+      for (int j = swc::PW - 1; j >= 0; --j) {
+        const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+        ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
       }
       if (lane < nc) X[(int64_t)r * ldx + pn.c0 + lane] = ti;
     }
