@@ -71,11 +71,12 @@ __device__ inline int find_pos(const int* a, int len, int v) {
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
-// Lane layout of v_mfma_f64_16x16x4f64 on gfx950 (fixed by an on-device probe):
-// lane l feeds A[l%16][k0 + l/16] and B^T[k0 + l/16][l%16] and receives
-// D[l/16 + 4r][l%16] in component r.
+// Lane layout of the 16x16x4 MFMA builtins on gfx950 (fixed by an on-device
+// probe of both): lane l feeds A[l%16][k0 + l/16] and B^T[k0 + l/16][l%16];
+// v_mfma_f64_16x16x4f64 leaves D[l/16 + 4r][l%16] in component r, the f32
+// variant D[4(l/16) + r][l%16].
 
 // Raw buffer resource over [base, base + bytes) (gfx9 descriptor word 3): a
 // buffer load whose byte offset falls outside returns zero.
@@ -86,10 +87,14 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t byt
 
 // The whole offset goes in the VGPR operand: the range check covers only the
 // VGPR offset (plus the instruction offset), not the SGPR one.
-__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+template <typename T>
+__device__ inline T load_elem(__amdgpu_buffer_rsrc_t r, unsigned voff);
+template <>
+__device__ inline double load_elem<double>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
 }
-__device__ inline float load_f32(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+template <>
+__device__ inline float load_elem<float>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
 }
 
@@ -97,7 +102,7 @@ __device__ inline float load_f32(__amdgpu_buffer_rsrc_t r, unsigned voff) {
 // column), elements of es bytes: the e-th set bit of mask receives source row
 // base + e.  Positions receiving nothing get an offset past every buffer, so
 // their loads read zero.
-__device__ inline unsigned landing_off(uint64_t mask, int p, int base, int es) {
+__device__ inline unsigned landing_off(uint64_t mask, int p, int base, unsigned es) {
   return ((mask >> p) & 1ull) ? (unsigned)(base + __popcll(mask & ((1ull << p) - 1ull))) * es
                               : 0x80000000u;
 }
@@ -113,62 +118,68 @@ __device__ inline unsigned landing_off(uint64_t mask, int p, int base, int es) {
 // so rows that land nowhere (all rows of an inactive block) and columns past k
 // read as zero without branches: a branch around a load would make the
 // compiler wait for it at the join instead of overlapping all of them.
-// With f32 the operands come from the FP32 copy of L (rounded to FP32) and the
-// products are accumulated in FP64 on the same MFMA.
-__device__ inline void task_update(const void* __restrict__ src, bool f32, int ld, int k, int ra,
-                                   int ca, uint64_t rmask, uint64_t cmask, bool diag, int wave,
-                                   int lane, d4 (&acc)[4][4]) {
+// T = float reads the FP32 copy of L and multiplies on v_mfma_f32_16x16x4f32,
+// whose A rows are fed in the order pi(x) = x/4 + 4(x%4) so that its output
+// (D[4(l/16) + r][l%16]) lands in the f64 layout; each 16-column chunk is
+// accumulated in FP32 and then added to the FP64 accumulator.
+template <typename T>
+__device__ inline void task_update(const T* __restrict__ src, int ld, int k, int ra, int ca,
+                                   uint64_t rmask, uint64_t cmask, bool diag, int wave, int lane,
+                                   d4 (&acc)[4][4]) {
+  constexpr bool kF32 = sizeof(T) == 4;
   const int lr = lane & 15, lk = lane >> 4;
-  const int es = f32 ? 4 : 8;
-  const unsigned kstep = (unsigned)ld * es;  // bytes between source columns
+  const int arow = kF32 ? (lr >> 2) + 4 * (lr & 3) : lr;
+  const unsigned kstep = (unsigned)ld * sizeof(T);  // bytes between source columns
   unsigned ao[4], bo[4];
   bool ab[4], bb[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    ao[b] = landing_off(rmask, 16 * b + lr, ra, es) + lk * kstep;
-    bo[b] = landing_off(cmask, 16 * b + lr, ca, es) + lk * kstep;
+    ao[b] = landing_off(rmask, 16 * b + arow, ra, sizeof(T)) + lk * kstep;
+    bo[b] = landing_off(cmask, 16 * b + lr, ca, sizeof(T)) + lk * kstep;
     ab[b] = ((rmask >> (16 * b)) & 0xffffull) != 0;
     bb[b] = ((cmask >> (16 * b)) & 0xffffull) != 0;
   }
   for (int kb = 16 * wave; kb < k; kb += 64) {
     const __amdgpu_buffer_rsrc_t rs = make_rsrc((const char*)src + (int64_t)kb * kstep,
                                                 (int64_t)(k - kb) * kstep);
-    double a[4][4], bv[4][4];
-    if (f32) {
-      // all loads first, conversions after, so that the loads overlap
-      float af[4][4], bf[4][4];
+    T a[4][4], bv[4][4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          af[b][u] = load_f32(rs, ao[b] + 4 * u * kstep);
-          bf[b][u] = load_f32(rs, bo[b] + 4 * u * kstep);
-        }
+      for (int b = 0; b < 4; ++b) {
+        a[b][u] = load_elem<T>(rs, ao[b] + 4 * u * kstep);
+        bv[b][u] = load_elem<T>(rs, bo[b] + 4 * u * kstep);
+      }
+    if constexpr (!kF32) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          a[b][u] = (double)af[b][u];
-          bv[b][u] = (double)bf[b][u];
-        }
+        for (int bj = 0; bj < 4; ++bj)
+          if (ab[bi] && bb[bj] && !(diag && bi < bj)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][u], bv[bj][u], acc[bi][bj], 0, 0, 0);
+          }
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int bi = 0; bi < 4; ++bi) {
+        if (!ab[bi]) continue;
+        f4 c[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          a[b][u] = load_f64(rs, ao[b] + 4 * u * kstep);
-          bv[b][u] = load_f64(rs, bo[b] + 4 * u * kstep);
-        }
+        for (int bj = 0; bj < 4; ++bj) c[bj] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+          if (bb[bj] && !(diag && bi < bj)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              c[bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bi][u], bv[bj][u], c[bj], 0, 0, 0);
+          }
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bj][r];
+      }
     }
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
-        if (ab[bi] && bb[bj] && !(diag && bi < bj)) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][u], bv[bj][u], acc[bi][bj], 0, 0, 0);
-        }
   }
 }
 
@@ -189,10 +200,14 @@ __device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
 // are combined in a fixed order, (w0 + w1) + (w2 + w3), through LDS and
 // subtracted from the tile, so results do not depend on scheduling.  The
 // factorization of panels completed by this level follows in k_panel_factor.
+// MODE 0: FP64 throughout; 1: descendant (kind 0) tiles on the FP32 copy,
+// intra-supernode tiles in FP64 (a tile's tasks all have one kind); 2: FP32
+// for every tile.
+template <int MODE>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
               const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
-              int fp32_mask, double* __restrict__ dscratch, const float* __restrict__ Lf) {
+              double* __restrict__ dscratch, const float* __restrict__ Lf) {
   __shared__ double red[2][swc::PW * kLdsLd];
   const Tile tl = tiles[tile0 + blockIdx.x];
   const Panel pn = panels[tl.panel];
@@ -207,11 +222,20 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
   for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int t = tl.task_begin; t < tl.task_end; ++t) {
-    const Task tk = tasks[t];
-    const bool f32 = (fp32_mask >> tk.kind) & 1;
-    const void* src = f32 ? (const void*)(Lf + tk.src_base) : (const void*)(L + tk.src_base);
-    task_update(src, f32, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask, diag, wave, lane, acc);
+  const bool f32 = MODE == 2 || (MODE == 1 && tl.task_end > tl.task_begin &&
+                                   tasks[tl.task_begin].kind == 0);
+  if (f32) {
+    for (int t = tl.task_begin; t < tl.task_end; ++t) {
+      const Task tk = tasks[t];
+      task_update<float>(Lf + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask, diag,
+                         wave, lane, acc);
+    }
+  } else {
+    for (int t = tl.task_begin; t < tl.task_end; ++t) {
+      const Task tk = tasks[t];
+      task_update<double>(L + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask, diag,
+                          wave, lane, acc);
+    }
   }
   // partial sums (w0 + w1) and (w2 + w3), then their sum, element-wise by all
   // threads: thread t owns elements t, t + 256, ... of the column-major tile
@@ -1092,11 +1116,13 @@ void enqueue_factor(swc_handle* h, bool profile) {
     const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
     hipEvent_t* pe = profile ? &h->pev[(size_t)4 * l] : nullptr;
     if (pe) hipEventRecord(pe[0], st);
-    if (nt > 0)
-      hipLaunchKernelGGL(k_tile_update, dim3(nt), dim3(kThreads), 0, st, h->dL,
-                         (const Panel*)h->dPanels, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0, h->fp32_mask, h->dDiag,
+    if (nt > 0) {
+      auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
+                                                                             : k_tile_update<2>);
+      hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, h->dL, (const Panel*)h->dPanels,
+                         (const Tile*)h->dTiles, (const Task*)h->dTasks, t0, h->dDiag,
                          (const float*)h->dLf);
+    }
     if (pe) hipEventRecord(pe[1], st);
     if (ns > 0) {
       // the level's panels by width class: <= 16, <= 32, <= 64 columns
