@@ -158,15 +158,17 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[11] = {panels, tiles, tasks, levels, n,
- * xsize, factor-list tiles, forward solve blocks, contribution-vector size, 0,
- * diagonal-block scratch doubles}; swc_plan_export copies array `which` (0
- * panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4 level->factor-list
- * offsets, 5 panels by level, 6 level->panel offsets, 7 column->gather offsets
- * (int64, n+1), 8 gathered contribution offsets (int64, vsize), 9 factor-list
- * tiles, 10 forward solve blocks, 11 level->solve-block offsets, 12 gathered
- * columns by level, 13 level->gathered-column offsets) as the packed structs
- * and arrays of swcholesky_amd/csrc/plan.h. */
+ * device, for tools and tests.  sizes[12] = {panels, tiles, tasks, levels, n,
+ * xsize, factor-list tiles, solve row blocks, contribution-vector size, solve
+ * panels, diagonal-block scratch doubles, solve levels}; swc_plan_export copies
+ * array `which` (0 panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4
+ * level->factor-list offsets, 5 panels by level, 6 level->panel offsets, 7
+ * column->gather offsets (int64, n+1), 8 gathered contribution offsets (int64,
+ * vsize), 9 factor-list tiles, 10 solve row blocks, 11 solve level->row-block
+ * offsets, 12 gathered columns by solve level, 13 solve level->gathered-column
+ * offsets, 14 solve panels (Panel structs), 15 solve panels by level, 16 solve
+ * level->solve-panel offsets) as the packed structs and arrays of
+ * swcholesky_amd/csrc/plan.h. */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,

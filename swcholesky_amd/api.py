@@ -266,26 +266,27 @@ def build_plan(bcsc: BCSC):
     check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
                              ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
     try:
-        sz = np.zeros(11, dtype=np.int64)
+        sz = np.zeros(12, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, _, dscr = (
+        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, nsp, dscr, nslevel = (
             int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
                 (6, np.int32, nlevel + 1), (7, np.int64, n + 1), (8, np.int64, vsize),
-                (9, np.int32, ntrsm), (10, SBLK_DT, nfblk), (11, np.int32, nlevel + 1),
-                (12, np.int32, n), (13, np.int32, nlevel + 1)]
+                (9, np.int32, ntrsm), (10, SBLK_DT, nfblk), (11, np.int32, nslevel + 1),
+                (12, np.int32, n), (13, np.int32, nslevel + 1), (14, PANEL_DT, nsp),
+                (15, np.int32, nsp), (16, np.int32, nslevel + 1)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
                  "lvl_panels", "gptr", "gidx", "trsm_tiles", "fblocks", "lvl_fblk",
-                 "gcols", "lvl_gcol"]
+                 "gcols", "lvl_gcol", "spanels", "sp_order", "lvl_sp"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)
             check(lib.swc_plan_export(h, which, ptr(a)))
             out[name] = a
-        out.update(nlevel=nlevel, xsize=xsize, update_flops=fl[0], factor_flops=fl[1],
-                   vsize=vsize, diag_scratch=dscr)
+        out.update(nlevel=nlevel, nslevel=nslevel, xsize=xsize, update_flops=fl[0],
+                   factor_flops=fl[1], vsize=vsize, diag_scratch=dscr)
         return out
     finally:
         lib.swc_plan_free(h)

@@ -496,60 +496,92 @@ k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __res
   }
 }
 
-// One forward row block: solve the panel's diagonal block (every block of the
-// panel, from the same inputs), write the solution if this is the panel's
-// writer block, then push L21 x of its rows.
+// One forward row block of a solve panel (up to SPW = 256 columns of one
+// supernode): solve the panel's diagonal block (every block of the panel, from
+// the same inputs) in 64-column chunks -- wave 0 substitutes the chunk's 64x64
+// triangle from registers, then all threads subtract its products from the
+// panel's later rows -- write the solution if this is the panel's writer
+// block, then push L21 x of the block's rows.
 __global__ void __launch_bounds__(kThreads)
-k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ panels,
+k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
             int64_t vld, double* __restrict__ X, double* __restrict__ Y, int nrhs, int64_t ldx,
             int64_t ldy) {
-  __shared__ double xs[swc::PW];
+  __shared__ double xs[swc::SPW];
   __shared__ double part[kThreads / 64][swc::TM];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
-  const Panel pn = panels[bk.panel];
+  const Panel sp = spanels[bk.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = pn.ncol;
-  // wave 0 holds row `lane` of L11 (zero beyond nc) and the reciprocal of its
-  // diagonal entry (1 beyond nc, which leaves the padded unknowns at zero)
-  double row[swc::PW];
-  double rinv = 1.0;
-  if (wave == 0) {
-#pragma unroll
-    for (int c = 0; c < swc::PW; ++c)
-      row[c] = (lane < nc && c < lane) ? L[pn.vbase + (int64_t)c * pn.ld + lane] : 0.0;
-    if (lane < nc) rinv = 1.0 / L[pn.vbase + (int64_t)lane * pn.ld + lane];
-  }
-  const double* blk = L + pn.vbase + bk.r0;
+  const int nc = sp.ncol, nch = (nc + 63) / 64;
+  const double* D = L + sp.vbase;  // the panel's diagonal block, column j at D + j * ld
   for (int r = 0; r < nrhs; ++r) {
     double* x = X + (int64_t)r * ldx;
     double* v = V + (int64_t)r * vld;
-    if (wave == 0) {
-      double ti = lane < nc ? x[pn.c0 + lane] : 0.0;
-      // column-oriented substitution: x_j = t_j / L_jj, then t_i -= L_ij x_j
-#pragma unroll
-      for (int j = 0; j < swc::PW; ++j) {
-        const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-        ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
-      }
-      if (lane < nc) {
-        xs[lane] = ti;
-        if (bk.writer) Y[(int64_t)r * ldy + pn.c0 + lane] = ti;
-      }
-    }
+    for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
     __syncthreads();
+    for (int a = 0; a < nch; ++a) {
+      const int ca = 64 * a, na = min(64, nc - ca);
+      if (wave == 0) {
+        // row `lane` of the chunk's triangle (zero beyond na) and the
+        // reciprocal of its diagonal entry (1 beyond na: padded unknowns stay 0)
+        double row[64];
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+          row[c] = (lane < na && c < lane) ? D[(int64_t)(ca + c) * sp.ld + ca + lane] : 0.0;
+        const double rinv = lane < na ? 1.0 / D[(int64_t)(ca + lane) * sp.ld + ca + lane] : 1.0;
+        double ti = lane < na ? xs[ca + lane] : 0.0;
+        // column-oriented substitution: x_j = t_j / L_jj, then t_i -= L_ij x_j
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+          const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+          ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
+        }
+        if (lane < na) xs[ca + lane] = ti;
+      }
+      __syncthreads();
+      const int i = ca + 64 + tid;  // a later row of the diagonal block
+      if (i < nc) {
+        double acc = 0.0;
+        for (int c0 = 0; c0 < na; c0 += 16) {  // 16 loads in flight per step
+          double lv[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            lv[u] = c0 + u < na ? D[(int64_t)(ca + c0 + u) * sp.ld + i] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc = fma(lv[u], xs[ca + c0 + u], acc);
+        }
+        xs[i] -= acc;
+      }
+      __syncthreads();
+    }
+    if (bk.writer)
+      for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
     if (bk.nr > 0) {
       // wave w sums columns j = w, w+4, ... for row `lane`
+      const double* blk = L + sp.vbase + bk.r0;
       double acc = 0.0;
-      if (lane < bk.nr)
-        for (int j = wave; j < nc; j += kThreads / 64) acc = fma(blk[(int64_t)j * pn.ld + lane], xs[j], acc);
+      if (lane < bk.nr) {
+        for (int q0 = 0; wave + 4 * q0 < nc; q0 += 16) {  // 16 loads in flight per step
+          double lv[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int j = wave + 4 * (q0 + u);
+            lv[u] = j < nc ? blk[(int64_t)j * sp.ld + lane] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int j = wave + 4 * (q0 + u);
+            acc = fma(lv[u], j < nc ? xs[j] : 0.0, acc);
+          }
+        }
+      }
       part[wave][lane] = acc;
       __syncthreads();
       if (wave == 0 && lane < bk.nr) {
         const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-        const int p = (pn.c0 - pn.sc0) + bk.r0 + lane;  // position in the supernode's rows
-        if (p < pn.sw) x[pn.sc0 + p] -= y;               // a later column of the supernode
-        else v[pn.voff + (p - pn.sw)] += y;              // a row of an ancestor
+        const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;  // position in the supernode's rows
+        if (p < sp.sw) x[sp.sc0 + p] -= y;               // a later column of the supernode
+        else v[sp.voff + (p - sp.sw)] += y;              // a row of an ancestor
       }
     }
     __syncthreads();
@@ -557,85 +589,112 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ panels,
 }
 
 // Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
-// for the panel's columns j, into Pb at the block's slot in its level.  Wave w
-// owns columns j = w + 4q, lane l row l of the block; each column's lane
+// for the solve panel's columns j, into Pb at the block's slot in its level.
+// Wave w owns columns j = w + 4q, lane l row l of the block; each column's lane
 // products are combined by a fixed butterfly (deterministic).
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
-              const Panel* __restrict__ panels, const swc::SolveBlock* __restrict__ blocks,
+              const Panel* __restrict__ spanels, const swc::SolveBlock* __restrict__ blocks,
               int b0, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
-  const Panel pn = panels[bk.panel];
+  const Panel sp = spanels[bk.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = pn.ncol;
-  const double* blk = L + pn.vbase + bk.r0;
-  const int row = lane < bk.nr ? Ls[pn.rows_off + bk.r0 + lane] : 0;
+  const int nc = sp.ncol;
+  const double* blk = L + sp.vbase + bk.r0;
+  const int row = lane < bk.nr ? Ls[sp.rows_off + bk.r0 + lane] : 0;
   for (int r = 0; r < nrhs; ++r) {
     const double xr = lane < bk.nr ? X[(int64_t)r * ldx + row] : 0.0;
-    double* out = Pb + (int64_t)r * pld + (int64_t)blockIdx.x * swc::PW;
-    double a[16];
+    double* out = Pb + (int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW;
+    for (int q0 = 0; q0 < swc::SPW / 4; q0 += 16) {
+      if (wave + 4 * q0 >= nc) break;
+      double a[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int j = wave + 4 * q;
-      a[q] = (j < nc && lane < bk.nr) ? blk[(int64_t)j * pn.ld + lane] * xr : 0.0;
-    }
+      for (int q = 0; q < 16; ++q) {
+        const int j = wave + 4 * (q0 + q);
+        a[q] = (j < nc && lane < bk.nr) ? blk[(int64_t)j * sp.ld + lane] * xr : 0.0;
+      }
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < 16; ++q) {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
-      const int j = wave + 4 * q;
-      if (lane == 0 && j < nc) out[j] = a[q];
+        for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
+        const int j = wave + 4 * (q0 + q);
+        if (lane == 0 && j < nc) out[j] = a[q];
+      }
     }
   }
 }
 
-// Backward panel: t = y_p - (sum of its row blocks' partials), x_p = L11^{-T} t.
-// The partials are summed by wave w over blocks w, w+4, ... and the four sums
-// combined in a fixed order.
+// Backward solve panel: t = y_p - (sum of its row blocks' partials, in block
+// order), then L11^T x_p = t in 64-column chunks from the last: wave 0
+// substitutes the chunk's transposed triangle from registers, then all threads
+// subtract its products from the panel's earlier entries.
 __global__ void __launch_bounds__(kThreads)
-k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ panels,
+k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
            const int* __restrict__ order, int p0, const int* __restrict__ fblk_ptr, int b0,
            const double* __restrict__ Pb, int64_t pld, const double* __restrict__ Y,
            double* __restrict__ X, int nrhs, int64_t ldx, int64_t ldy) {
-  __shared__ double dl[swc::PW * kLdsLd];
-  __shared__ double part[kThreads / 64][swc::PW];
+  __shared__ double ts[swc::SPW];
   const int qi = p0 + blockIdx.x;
-  const Panel pn = panels[order[qi]];
+  const Panel sp = spanels[order[qi]];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = pn.ncol;
+  const int nc = sp.ncol, nch = (nc + 63) / 64;
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
-  load_diag(L, pn, dl);
-  __syncthreads();
-  // wave 0 holds column `lane` of L11 below the diagonal (zero beyond nc) and
-  // the reciprocal of its diagonal entry
-  double col[swc::PW];
-  double rinv = 1.0;
-  if (wave == 0) {
-#pragma unroll
-    for (int j = 0; j < swc::PW; ++j) col[j] = (lane < nc && j > lane && j < nc) ? dl[lane * kLdsLd + j] : 0.0;
-    if (lane < nc) rinv = 1.0 / dl[lane * kLdsLd + lane];
-  }
+  const double* D = L + sp.vbase;
   for (int r = 0; r < nrhs; ++r) {
     const double* pb = Pb + (int64_t)r * pld;
-    double sum = 0.0;
-    if (lane < nc)
-      for (int b = f0 + wave; b < f1; b += kThreads / 64) sum += pb[(int64_t)(b - b0) * swc::PW + lane];
-    part[wave][lane] = sum;
-    __syncthreads();
-    if (wave == 0) {
-      double ti = 0.0;
-      if (lane < nc)
-        ti = Y[(int64_t)r * ldy + pn.c0 + lane] -
-             (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
-      // x_j = t_j / L_jj, then t_i -= L_ji x_j for i < j
+    for (int c = tid; c < nc; c += kThreads) {
+      double sum = 0.0;
+      for (int b = f0; b < f1; b += 8) {  // 8 loads in flight per step, summed in order
+        double pv[8];
 #pragma unroll
-      for (int j = swc::PW - 1; j >= 0; --j) {
-        const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-        ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
+        for (int u = 0; u < 8; ++u) pv[u] = b + u < f1 ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum += pv[u];
       }
-      if (lane < nc) X[(int64_t)r * ldx + pn.c0 + lane] = ti;
+      ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - sum;
     }
+    __syncthreads();
+    for (int a = nch - 1; a >= 0; --a) {
+      const int ca = 64 * a, na = min(64, nc - ca);
+      if (wave == 0) {
+        // column `lane` of the chunk's triangle below the diagonal (zero beyond
+        // na) and the reciprocal of its diagonal entry
+        double col[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+          col[j] = (lane < na && j > lane && j < na) ? D[(int64_t)(ca + lane) * sp.ld + ca + j] : 0.0;
+        const double rinv = lane < na ? 1.0 / D[(int64_t)(ca + lane) * sp.ld + ca + lane] : 1.0;
+        double ti = lane < na ? ts[ca + lane] : 0.0;
+        // x_j = t_j / L_jj, then t_i -= L_ji x_j for i < j
+#pragma unroll
+        for (int j = 63; j >= 0; --j) {
+          const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+          ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
+        }
+        if (lane < na) ts[ca + lane] = ti;
+      }
+      __syncthreads();
+      // earlier entries: t_i -= sum_j L(ca + j, i) x_(ca + j), i < ca; wave w
+      // takes i = w, w+4, ..., lane j row ca + j (coalesced), butterfly sums
+      for (int i0 = wave; i0 < ca; i0 += 4 * 8) {
+        double pr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + 4 * u;
+          pr[u] = (i < ca && lane < na) ? D[(int64_t)i * sp.ld + ca + lane] * ts[ca + lane] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) pr[u] += __shfl_xor(pr[u], off);
+          const int i = i0 + 4 * u;
+          if (lane == 0 && i < ca) ts[i] -= pr[u];
+        }
+      }
+      __syncthreads();
+    }
+    for (int c = tid; c < nc; c += kThreads) X[(int64_t)r * ldx + sp.c0 + c] = ts[c];
     __syncthreads();
   }
 }
@@ -933,7 +992,9 @@ struct swc_handle {
   swc::SolveBlock* dFblk = nullptr;
   int* dFblkPtr = nullptr;  // per panel (level order): its forward/backward row blocks
   double* dV = nullptr;   // contribution vectors, vsize * work_cols
-  double* dPb = nullptr;  // backward partial sums, max_lvl_fblk * PW * work_cols
+  double* dPb = nullptr;  // backward partial sums, max_lvl_fblk * SPW * work_cols
+  Panel* dSpanels = nullptr;  // solve panels
+  int* dSpOrder = nullptr;    // solve panels by level
   double* dY = nullptr;   // forward solutions, n * work_cols
   int* dOrder = nullptr;
   int* dTrsm = nullptr;
@@ -1002,14 +1063,14 @@ struct swc_handle {
   void release() {
     drop_graphs();
     hipFree(dL); hipFree(dLf); hipFree(dLs); hipFree(dPanels); hipFree(dTiles); hipFree(dTasks);
-    hipFree(dGptr); hipFree(dGidx); hipFree(dGcols); hipFree(dFblk); hipFree(dFblkPtr); hipFree(dV); hipFree(dPb); hipFree(dY); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
+    hipFree(dGptr); hipFree(dGidx); hipFree(dGcols); hipFree(dFblk); hipFree(dFblkPtr); hipFree(dV); hipFree(dPb); hipFree(dY); hipFree(dSpanels); hipFree(dSpOrder); hipFree(dOrder); hipFree(dTrsm); hipFree(dDiag); hipFree(dAmap); hipFree(dPerm);
     hipFree(dVals); hipFree(dWork); hipFree(dInfo);
     hipFree(dRp); hipFree(dCi); hipFree(dVmap); hipFree(dAf); hipFree(dScal); hipFree(dPart); hipFree(dNormPart);
     hipFree(dRef);
     hipFree(dZ); hipFree(dArena); hipFree(dNodes); hipFree(dCol2sup); hipFree(dSpx);
     hipFree(dSsuper); hipFree(dSpi); hipFree(dDiagPos); hipFree(dQi); hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     dRef = nullptr;
@@ -1091,7 +1152,7 @@ int ensure_work(swc_handle* h, int cols) {
   SWC_HIP(hipMalloc((void**)&h->dWork, sizeof(double) * (size_t)h->n * cols));
   SWC_HIP(hipMalloc((void**)&h->dV, sizeof(double) * (size_t)std::max<int64_t>(P.vsize, 1) * cols));
   SWC_HIP(hipMalloc((void**)&h->dY, sizeof(double) * (size_t)h->n * cols));
-  SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_fblk, 1) * swc::PW * cols));
+  SWC_HIP(hipMalloc((void**)&h->dPb, sizeof(double) * (size_t)std::max(P.max_lvl_fblk, 1) * swc::SPW * cols));
   h->work_cols = cols;
   return SWC_OK;
 }
@@ -1215,8 +1276,9 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
   const int64_t ldy = h->n;
+  const Panel* sps = (const Panel*)h->dSpanels;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
-  for (int l = 0; l < P.nlevel; ++l) {
+  for (int l = 0; l < P.nslevel; ++l) {
     const int g0 = P.lvl_gcol[l], ng = P.lvl_gcol[l + 1] - g0;
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
     if (ng > 0)
@@ -1224,24 +1286,22 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const int*)h->dGcols, g0, ng, (const int64_t*)h->dGptr,
                          (const int64_t*)h->dGidx, (const double*)h->dV, vld, dX, nrhs, ldx);
     if (nb > 0)
-      hipLaunchKernelGGL(k_fwd_panel, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const Panel*)h->dPanels, (const swc::SolveBlock*)h->dFblk, b0, h->dV,
-                         vld, dX, h->dY, nrhs, ldx, ldy);
+      hipLaunchKernelGGL(k_fwd_panel, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+                         (const swc::SolveBlock*)h->dFblk, b0, h->dV, vld, dX, h->dY, nrhs, ldx,
+                         ldy);
   }
-  const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::PW;
-  for (int l = P.nlevel - 1; l >= 0; --l) {
-    const int p0 = P.lvl_panels[l], np = P.lvl_panels[l + 1] - p0;
+  const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
+  for (int l = P.nslevel - 1; l >= 0; --l) {
+    const int p0 = P.lvl_sp[l], np = P.lvl_sp[l + 1] - p0;
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
     if (nb > 0)
       hipLaunchKernelGGL(k_bwd_partial, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, (const Panel*)h->dPanels,
-                         (const swc::SolveBlock*)h->dFblk, b0, h->dPb, pld, (const double*)dX,
-                         nrhs, ldx);
+                         (const int*)h->dLs, sps, (const swc::SolveBlock*)h->dFblk, b0, h->dPb,
+                         pld, (const double*)dX, nrhs, ldx);
     if (np > 0)
-      hipLaunchKernelGGL(k_bwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const Panel*)h->dPanels, (const int*)h->dOrder, p0,
-                         (const int*)h->dFblkPtr, b0, (const double*)h->dPb, pld,
-                         (const double*)h->dY, dX, nrhs, ldx, ldy);
+      hipLaunchKernelGGL(k_bwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+                         (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
+                         (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
 }
 
@@ -1628,6 +1688,8 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dGcols, P.gcols.data(), P.gcols.size(), st);
   if (!rc) rc = upload(&h->dFblk, P.fblocks.data(), P.fblocks.size(), st);
   if (!rc) rc = upload(&h->dFblkPtr, P.fblk_ptr.data(), P.fblk_ptr.size(), st);
+  if (!rc) rc = upload(&h->dSpanels, P.spanels.data(), P.spanels.size(), st);
+  if (!rc) rc = upload(&h->dSpOrder, P.sp_order.data(), P.sp_order.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
   if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
   if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);

@@ -213,35 +213,76 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
   }
 
-  // ---- forward solve blocks and gathered columns, by level ----
-  P.lvl_fblk.assign(P.nlevel + 1, 0);
-  P.fblk_ptr.assign(P.npanel + 1, 0);
-  for (int l = 0; l < P.nlevel; ++l) {
-    for (int qi = P.lvl_panels[l]; qi < P.lvl_panels[l + 1]; ++qi) {
-      const int q = P.panel_order[qi];
-      const Panel& pn = P.panels[q];
-      P.fblk_ptr[qi] = (int)P.fblocks.size();
-      if (pn.nrow == pn.ncol) {
-        P.fblocks.push_back({q, pn.ncol, 0, 1});
-      } else {
-        for (int r0 = pn.ncol; r0 < pn.nrow; r0 += TM)
-          P.fblocks.push_back({q, r0, std::min(TM, pn.nrow - r0), r0 == pn.ncol ? 1 : 0});
-      }
-    }
-    P.lvl_fblk[l + 1] = (int)P.fblocks.size();
-    P.max_lvl_fblk = std::max(P.max_lvl_fblk, P.lvl_fblk[l + 1] - P.lvl_fblk[l]);
-  }
-  P.fblk_ptr[P.npanel] = (int)P.fblocks.size();
-  P.lvl_gcol.assign(P.nlevel + 1, 0);
-  for (int s = 0; s < nsuper; ++s)
-    P.lvl_gcol[P.panel_level[first_panel[s]] + 1] += super[s + 1] - super[s];
-  for (int l = 0; l < P.nlevel; ++l) P.lvl_gcol[l + 1] += P.lvl_gcol[l];
-  P.gcols.resize(n);
+  // ---- solve panels: up to SPW columns of one supernode; their own levels ----
+  // A supernode's first solve panel follows the last solve panel of every
+  // supernode in its pruned set (its descendants); its later ones follow it.
   {
-    std::vector<int> fill(P.lvl_gcol.begin(), P.lvl_gcol.end() - 1);
+    std::vector<int> last_sp_level(nsuper, 0);
+    std::vector<int> sp_level;
     for (int s = 0; s < nsuper; ++s) {
-      int& f = fill[P.panel_level[first_panel[s]]];
-      for (int c = super[s]; c < super[s + 1]; ++c) P.gcols[f++] = c;
+      int first = 0;
+      for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t)
+        first = std::max(first, last_sp_level[pruneSet[t]] + 1);
+      const int W = super[s + 1] - super[s];
+      const int64_t R = pi[s + 1] - pi[s];
+      int lvl = first;
+      for (int c = 0; c < W; c += SPW, ++lvl) {
+        Panel sp;
+        sp.vbase = px[s] + (int64_t)c * R + c;
+        sp.ld = (int)R;
+        sp.ncol = std::min(SPW, W - c);
+        sp.nrow = (int)R - c;
+        sp.c0 = super[s] + c;
+        sp.rows_off = (int)(pi[s] + c);
+        sp.sn = s;
+        sp.voff = voff[s];
+        sp.sc0 = super[s];
+        sp.sw = W;
+        P.spanels.push_back(sp);
+        sp_level.push_back(lvl);
+      }
+      last_sp_level[s] = lvl - 1;
+      P.nslevel = std::max(P.nslevel, lvl);
+    }
+    const int nsp = (int)P.spanels.size();
+    P.lvl_sp.assign(P.nslevel + 1, 0);
+    for (int q = 0; q < nsp; ++q) P.lvl_sp[sp_level[q] + 1]++;
+    for (int l = 0; l < P.nslevel; ++l) P.lvl_sp[l + 1] += P.lvl_sp[l];
+    P.sp_order.resize(nsp);
+    std::vector<int> fill(P.lvl_sp.begin(), P.lvl_sp.end() - 1);
+    for (int q = 0; q < nsp; ++q) P.sp_order[fill[sp_level[q]]++] = q;
+
+    // row blocks below each solve panel's diagonal block, by level
+    P.lvl_fblk.assign(P.nslevel + 1, 0);
+    P.fblk_ptr.assign(nsp + 1, 0);
+    for (int l = 0; l < P.nslevel; ++l) {
+      for (int qi = P.lvl_sp[l]; qi < P.lvl_sp[l + 1]; ++qi) {
+        const int q = P.sp_order[qi];
+        const Panel& sp = P.spanels[q];
+        P.fblk_ptr[qi] = (int)P.fblocks.size();
+        if (sp.nrow == sp.ncol) {
+          P.fblocks.push_back({q, sp.ncol, 0, 1});
+        } else {
+          for (int r0 = sp.ncol; r0 < sp.nrow; r0 += TM)
+            P.fblocks.push_back({q, r0, std::min(TM, sp.nrow - r0), r0 == sp.ncol ? 1 : 0});
+        }
+      }
+      P.lvl_fblk[l + 1] = (int)P.fblocks.size();
+      P.max_lvl_fblk = std::max(P.max_lvl_fblk, P.lvl_fblk[l + 1] - P.lvl_fblk[l]);
+    }
+    P.fblk_ptr[nsp] = (int)P.fblocks.size();
+
+    // columns gathered when their supernode's first solve panel starts
+    P.lvl_gcol.assign(P.nslevel + 1, 0);
+    std::vector<int> first_level(nsuper);
+    for (int q = nsp - 1; q >= 0; --q) first_level[P.spanels[q].sn] = sp_level[q];
+    for (int sn = 0; sn < nsuper; ++sn) P.lvl_gcol[first_level[sn] + 1] += super[sn + 1] - super[sn];
+    for (int l = 0; l < P.nslevel; ++l) P.lvl_gcol[l + 1] += P.lvl_gcol[l];
+    P.gcols.resize(n);
+    std::vector<int> gf(P.lvl_gcol.begin(), P.lvl_gcol.end() - 1);
+    for (int sn = 0; sn < nsuper; ++sn) {
+      int& f = gf[first_level[sn]];
+      for (int c = super[sn]; c < super[sn + 1]; ++c) P.gcols[f++] = c;
     }
   }
 
@@ -407,8 +448,9 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[6] = (int64_t)P.trsm_tiles.size();
   out[7] = (int64_t)P.fblocks.size();
   out[8] = P.vsize;
-  out[9] = 0;
+  out[9] = (int64_t)P.spanels.size();
   out[10] = P.diag_scratch;
+  out[11] = P.nslevel;
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
@@ -438,6 +480,9 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 11: return cp(P.lvl_fblk);
     case 12: return cp(P.gcols);
     case 13: return cp(P.lvl_gcol);
+    case 14: return cp(P.spanels);
+    case 15: return cp(P.sp_order);
+    case 16: return cp(P.lvl_sp);
     default: return swc_set_error(SWC_EINVAL, "swc_plan_export: unknown array id");
   }
 }

@@ -19,6 +19,7 @@ namespace swc {
 
 constexpr int PW = 64;  // panel width (columns)
 constexpr int TM = 64;  // tile height (rows)
+constexpr int SPW = 256;  // solve panel width (columns)
 
 struct Panel {
   int64_t vbase;  // value offset of (panel row 0, panel column 0)
@@ -62,9 +63,9 @@ struct Task {
   uint64_t cmask;    // panel columns receiving source rows ca..cb-1
 };
 
-// Forward-solve row block of a panel: rows [r0, r0+nr) below its diagonal
-// block (nr = 0: the panel has no rows below it).  The block with writer = 1
-// also stores the panel's solution.
+// Solve row block of a solve panel: rows [r0, r0+nr) below its diagonal block
+// (nr = 0: the panel has no rows below it).  In the forward sweep the block with
+// writer = 1 also stores the panel's solution.
 struct SolveBlock {
   int panel;
   int r0;
@@ -91,6 +92,12 @@ struct Plan {
   std::vector<int> panel_level;     // per panel
   std::vector<int> lvl_panels;      // nlevel+1 offsets into panel_order
   std::vector<int> panel_order;     // panels grouped by level
+  // solves: solve panels of up to SPW columns of one supernode, with their own
+  // level structure (nslevel levels)
+  std::vector<Panel> spanels;
+  std::vector<int> sp_order;        // solve panels grouped by level
+  std::vector<int> lvl_sp;          // nslevel+1
+  int nslevel = 0;
   // forward solve: column c (factor ordering) takes the contributions
   // V[gidx[gptr[c] .. gptr[c+1])] of its descendants, in increasing supernode
   // order; the columns of the supernodes whose first panel is at level l are
@@ -98,10 +105,10 @@ struct Plan {
   std::vector<int64_t> gptr;        // n+1
   std::vector<int64_t> gidx;        // vsize
   std::vector<int> gcols;           // n
-  std::vector<int> lvl_gcol;        // nlevel+1
-  std::vector<SolveBlock> fblocks;  // row blocks of each level's panels (both sweeps)
-  std::vector<int> lvl_fblk;        // nlevel+1
-  std::vector<int> fblk_ptr;        // npanel+1: blocks of panel_order[i]
+  std::vector<int> lvl_gcol;        // nslevel+1
+  std::vector<SolveBlock> fblocks;  // row blocks of each level's solve panels (both sweeps)
+  std::vector<int> lvl_fblk;        // nslevel+1
+  std::vector<int> fblk_ptr;        // nsp+1: blocks of sp_order[i]
   int max_lvl_fblk = 0;             // most row blocks in one level
   int64_t diag_scratch = 0;         // doubles of diagonal-block copies one level needs
   int64_t vsize = 0;                // sum over supernodes of (rows - width)

@@ -96,13 +96,14 @@ def factor(plan, Ls, xsize, amap, vals):
 
 def solve(plan, Ls, L, b):
     """Forward then backward solve in the factor ordering (b is an n-vector),
-    as k_fwd_gather / k_fwd_panel / k_bwd_panel run it: the forward sweep
+    as k_fwd_gather / k_fwd_panel / k_bwd_partial / k_bwd_diag run it over the
+    solve panels (up to 256 columns of one supernode): the forward sweep
     accumulates right-hand sides in x and writes the panel solutions to y; the
     backward sweep reads y and the final values of x, and overwrites x."""
     x = np.array(b, dtype=np.float64)
     n = len(x)
     y = np.full(n, np.nan)
-    panels, order, lp = plan["panels"], plan["panel_order"], plan["lvl_panels"]
+    panels, order, lp = plan["spanels"], plan["sp_order"], plan["lvl_sp"]
     gptr, gidx, gcols, lg = plan["gptr"], plan["gidx"], plan["gcols"], plan["lvl_gcol"]
     fb, lf = plan["fblocks"], plan["lvl_fblk"]
     vsize = plan["vsize"]
@@ -117,7 +118,7 @@ def solve(plan, Ls, L, b):
         _chk(int(idx.min()), int(idx.max()) + 1, xsize, "diag")
         return np.tril(L[idx])
 
-    for lvl in range(plan["nlevel"]):
+    for lvl in range(plan["nslevel"]):
         for c in gcols[lg[lvl]:lg[lvl + 1]]:
             for e in range(gptr[c], gptr[c + 1]):
                 x[c] -= V[gidx[e]]
@@ -126,7 +127,7 @@ def solve(plan, Ls, L, b):
             pn = panels[bk["panel"]]
             nc, ld, c0 = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"])
             r0, nr = int(bk["r0"]), int(bk["nr"])
-            assert 0 <= nr <= 64 and r0 >= nc and r0 + nr <= int(pn["nrow"])
+            assert 0 <= nr <= 64 and 1 <= nc <= 256 and r0 >= nc and r0 + nr <= int(pn["nrow"])
             xs = np.linalg.solve(diag(pn), x[c0:c0 + nc])
             if bk["writer"]:
                 y[c0:c0 + nc] = xs
@@ -144,7 +145,7 @@ def solve(plan, Ls, L, b):
                 _chk(int(vi.min()), int(vi.max()) + 1, vsize, "V push")
             V[vi] += yy[~intra]
     assert not np.isnan(y).any(), "a panel solution was never written"
-    for lvl in range(plan["nlevel"] - 1, -1, -1):
+    for lvl in range(plan["nslevel"] - 1, -1, -1):
         for qi in range(lp[lvl], lp[lvl + 1]):
             pn = panels[order[qi]]
             nc, ld, c0, nrow = int(pn["ncol"]), int(pn["ld"]), int(pn["c0"]), int(pn["nrow"])

@@ -59,3 +59,17 @@ def test_plan_covers_every_update_once():
     # the forward-solve panel order is sorted by level
     lp = plan["lvl_panels"]
     assert lp[0] == 0 and lp[-1] == len(panels) and np.all(np.diff(lp) >= 0)
+
+
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(20), lambda: P.vector3d_var(10)])
+def test_emulated_solve_wide_supernodes(make):
+    """Solve panels of several 256-column chunks (supernodes wider than 256)."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    assert b.maxSupWid > 256
+    plan = api.build_plan(b)
+    assert plan["nslevel"] < plan["nlevel"]
+    y = np.random.default_rng(2).uniform(-1, 1, b.n)
+    xe = E.solve(plan, b.s, Lo, y)
+    xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
+    assert relF(xe, xo) < 1e-12
