@@ -168,6 +168,24 @@ def main():
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     log(f"tile kernel: {tm['tile_launches']:.0f} launches, avg {avg_ms:.3f} ms, "
         f"{achieved:.2f} TFLOP/s; trsm {tm['trsm_ms']:.2f} ms")
+    # the largest fronts: the levels with the most update work (the external
+    # updates of the top separators), each one tile-kernel launch
+    lv = f.level_timings()
+    top = np.argsort(-lv[:, 1])[:3]
+    large = [{"level": int(l), "ms": float(lv[l, 0]), "gflop": float(lv[l, 1] / 1e9),
+              "tflops": float(lv[l, 1] / (lv[l, 0] * 1e-3) / 1e12),
+              "frac_fp64_peak": float(lv[l, 1] / (lv[l, 0] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS)}
+             for l in top if lv[l, 0] > 0]
+    for r in large:
+        log(f"large front: level {r['level']} {r['gflop']:.1f} GFLOP in {r['ms']:.2f} ms = "
+            f"{r['tflops']:.1f} TFLOP/s ({100 * r['frac_fp64_peak']:.0f}% of FP64 MFMA peak)")
+    traffic, traffic_src = None, None
+    pmc = os.path.join(ROOT, "profiles", "k_tile_update_lap3d64_pmc.json")
+    if args.side == 64 and os.path.exists(pmc):
+        with open(pmc) as fh:
+            rec = json.load(fh)
+        traffic = rec["hbm_bytes_per_launch"]
+        traffic_src = os.path.relpath(pmc, ROOT)
 
     cpu = None
     if not args.no_cpu:
@@ -197,7 +215,10 @@ def main():
         "paths": paths,
         "roofline": {"bound": "mfma", "kernel": "k_tile_update", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "hbm_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None},
+        "large_fronts": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -145,6 +145,11 @@ int swc_gradient(swc_handle* h, const double* x, int x_on_device, double alpha, 
                  double* grad, int grad_on_device);
 /* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
 int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
+/* Per factor level l < cap: out[2l] = duration (ms) of its tile-update launch in
+ * the last profiled factorization (swc_set_profiling), out[2l+1] = that
+ * launch's useful (lower-triangle) update flops; *nlevel = number of levels.
+ * No counterpart in the reference: instrumentation of the GPU schedule. */
+int swc_level_timings(const swc_handle* h, double* out, int cap, int* nlevel);
 /* With graphs on (default), swc_factor and the solves replay launch sequences
  * captured once per structure (hipGraph) instead of launching level by level. */
 int swc_set_graphs(swc_handle* h, int on);

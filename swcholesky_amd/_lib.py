@@ -37,6 +37,7 @@ SIGNATURES = {
     "swc_get_factor": (c_int, [c_vp, c_vp, c_int]),
     "swc_solve": (c_int, [c_vp, c_vp, c_int, c_i64, c_int]),
     "swc_plan_stats": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
+    "swc_level_timings": (c_int, [c_vp, c_vp, c_int, c_vp]),
     "swc_set_profiling": (c_int, [c_vp, c_int]),
     "swc_set_graphs": (c_int, [c_vp, c_int]),
     "swc_selected_inverse": (c_int, [c_vp, c_vp, c_int]),

@@ -227,6 +227,16 @@ class Factor:
                 "tile_flops", "trsm_flops"]
         return dict(zip(keys, out.tolist()))
 
+    def level_timings(self):
+        """(ms, useful flops) of each level's tile-update launch in the last
+        profiled factorization, as an nlevel x 2 array."""
+        lib = _lib.lib()
+        nl = ctypes.c_int(0)
+        check(lib.swc_level_timings(self._h, None, 0, ctypes.byref(nl)))
+        out = np.zeros((nl.value, 2))
+        check(lib.swc_level_timings(self._h, ptr(out), nl.value, ctypes.byref(nl)))
+        return out
+
     def plan_stats(self):
         out = np.zeros(5, dtype=np.int64)
         fl = (ctypes.c_double * 2)()

@@ -1050,6 +1050,7 @@ struct swc_handle {
   bool profile = false;
   std::vector<hipEvent_t> pev;  // per-launch pairs when profiling
   double t_factor_ms = 0, t_solve_ms = 0, t_tile_ms = 0, t_trsm_ms = 0;
+  std::vector<double> lvl_tile_ms;  // per level, when profiling
   int n_tile = 0, n_trsm = 0;
 
   ~swc_handle() { release(); }
@@ -1247,6 +1248,7 @@ int factor_device(swc_handle* h, const double* dvals) {
   if (h->profile) {
     h->t_tile_ms = h->t_trsm_ms = 0;
     h->n_tile = h->n_trsm = 0;
+    h->lvl_tile_ms.assign(P.nlevel, 0.0);
     for (int l = 0; l < P.nlevel; ++l) {
       const int nt = P.lvl_tiles[l + 1] - P.lvl_tiles[l];
       const int ns = P.lvl_trsm[l + 1] - P.lvl_trsm[l];
@@ -1254,6 +1256,7 @@ int factor_device(swc_handle* h, const double* dvals) {
       if (nt > 0) {
         SWC_HIP(hipEventElapsedTime(&ms, pe[0], pe[1]));
         h->t_tile_ms += ms;
+        h->lvl_tile_ms[l] = ms;
         h->n_tile++;
       }
       if (ns > 0) {
@@ -1938,6 +1941,19 @@ int swc_get_timings(const swc_handle* h, double* out) {
   out[5] = h->n_trsm;
   out[6] = P.tile_kernel_flops;
   out[7] = P.trsm_kernel_flops;
+  return SWC_OK;
+}
+
+int swc_level_timings(const swc_handle* h, double* out, int cap, int* nlevel) {
+  if (int rc = check_handle(h)) return rc;
+  if (!nlevel) return swc_set_error(SWC_EINVAL, "swc_level_timings: null argument");
+  const Plan& P = h->plan;
+  *nlevel = P.nlevel;
+  if (!out) return SWC_OK;
+  for (int l = 0; l < std::min(cap, P.nlevel); ++l) {
+    out[2 * l] = l < (int)h->lvl_tile_ms.size() ? h->lvl_tile_ms[l] : 0.0;
+    out[2 * l + 1] = P.lvl_tile_flops[l];
+  }
   return SWC_OK;
 }
 

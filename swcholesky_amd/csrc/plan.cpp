@@ -302,6 +302,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   });
   P.lvl_tiles.assign(P.nlevel + 1, 0);
   P.lvl_trsm.assign(P.nlevel + 1, 0);
+  P.lvl_tile_flops.assign(P.nlevel, 0.0);
   std::vector<std::vector<int>> trsm_by_level(P.nlevel);
   for (int oi : order) {
     PTile& pt = ptiles[oi];
@@ -333,6 +334,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
         pairs += t.rb - r;
       }
       P.tile_kernel_flops += 2.0 * pairs * t.k;
+      P.lvl_tile_flops[pt.level] += 2.0 * pairs * t.k;
     }
     pt.t.task_end = (int)P.tasks.size();
     P.lvl_tiles[pt.level + 1]++;

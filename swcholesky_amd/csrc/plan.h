@@ -116,6 +116,7 @@ struct Plan {
   // algorithmic flops of each kernel class: lower-triangle update products (tile
   // kernel); POTRF + TRSM of every panel, counted once (panel factor kernel)
   double tile_kernel_flops = 0, trsm_kernel_flops = 0;
+  std::vector<double> lvl_tile_flops;  // the tile kernel's useful flops per level
   int nlevel = 0;
 };
 
