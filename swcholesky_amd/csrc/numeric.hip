@@ -542,13 +542,12 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
       const int i = ca + 64 + tid;  // a later row of the diagonal block
       if (i < nc) {
         double acc = 0.0;
-        for (int c0 = 0; c0 < na; c0 += 16) {  // 16 loads in flight per step
-          double lv[16];
+        for (int c0 = 0; c0 < na; c0 += 32) {  // 32 loads in flight per step (coalesced over i)
+          double lv[32];
 #pragma unroll
-          for (int u = 0; u < 16; ++u)
-            lv[u] = c0 + u < na ? D[(int64_t)(ca + c0 + u) * sp.ld + i] : 0.0;
+          for (int u = 0; u < 32; ++u) lv[u] = c0 + u < na ? D[(int64_t)(ca + c0 + u) * sp.ld + i] : 0.0;
 #pragma unroll
-          for (int u = 0; u < 16; ++u) acc = fma(lv[u], xs[ca + c0 + u], acc);
+          for (int u = 0; u < 32; ++u) acc = fma(lv[u], xs[ca + c0 + u], acc);
         }
         xs[i] -= acc;
       }
@@ -590,38 +589,37 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
 
 // Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
 // for the solve panel's columns j, into Pb at the block's slot in its level.
-// Wave w owns columns j = w + 4q, lane l row l of the block; each column's lane
-// products are combined by a fixed butterfly (deterministic).
+// Thread j reads the block's rows of column j, contiguous in memory, all at
+// once, and sums them in row order against x(rows) staged in LDS.
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ spanels, const swc::SolveBlock* __restrict__ blocks,
               int b0, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
+  __shared__ double xr[swc::TM];
+  __shared__ int rows[swc::TM];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
   const Panel sp = spanels[bk.panel];
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = sp.ncol;
-  const double* blk = L + sp.vbase + bk.r0;
-  const int row = lane < bk.nr ? Ls[sp.rows_off + bk.r0 + lane] : 0;
+  const int tid = threadIdx.x;
+  const int nc = sp.ncol, nr = bk.nr;
+  const double* col = L + sp.vbase + bk.r0 + (int64_t)tid * sp.ld;  // column tid, block rows
+  if (tid < nr) rows[tid] = Ls[sp.rows_off + bk.r0 + tid];
+  double lv[swc::TM];
+  if (tid < nc) {
+#pragma unroll
+    for (int i = 0; i < swc::TM; ++i) lv[i] = i < nr ? col[i] : 0.0;
+  }
+  __syncthreads();
   for (int r = 0; r < nrhs; ++r) {
-    const double xr = lane < bk.nr ? X[(int64_t)r * ldx + row] : 0.0;
-    double* out = Pb + (int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW;
-    for (int q0 = 0; q0 < swc::SPW / 4; q0 += 16) {
-      if (wave + 4 * q0 >= nc) break;
-      double a[16];
+    if (tid < nr) xr[tid] = X[(int64_t)r * ldx + rows[tid]];
+    __syncthreads();
+    if (tid < nc) {
+      double acc = 0.0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int j = wave + 4 * (q0 + q);
-        a[q] = (j < nc && lane < bk.nr) ? blk[(int64_t)j * sp.ld + lane] * xr : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
-        const int j = wave + 4 * (q0 + q);
-        if (lane == 0 && j < nc) out[j] = a[q];
-      }
+      for (int i = 0; i < swc::TM; ++i) acc = fma(lv[i], i < nr ? xr[i] : 0.0, acc);
+      Pb[(int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW + tid] = acc;
     }
+    __syncthreads();
   }
 }
 
@@ -675,22 +673,16 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
         if (lane < na) ts[ca + lane] = ti;
       }
       __syncthreads();
-      // earlier entries: t_i -= sum_j L(ca + j, i) x_(ca + j), i < ca; wave w
-      // takes i = w, w+4, ..., lane j row ca + j (coalesced), butterfly sums
-      for (int i0 = wave; i0 < ca; i0 += 4 * 8) {
-        double pr[8];
+      // earlier entries: t_i -= sum_j L(ca + j, i) x_(ca + j) for i < ca; thread
+      // i reads the chunk's rows of column i, contiguous in memory, all at once
+      if (tid < ca) {
+        double lv[64];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int i = i0 + 4 * u;
-          pr[u] = (i < ca && lane < na) ? D[(int64_t)i * sp.ld + ca + lane] * ts[ca + lane] : 0.0;
-        }
+        for (int j = 0; j < 64; ++j) lv[j] = j < na ? D[(int64_t)tid * sp.ld + ca + j] : 0.0;
+        double acc = 0.0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) pr[u] += __shfl_xor(pr[u], off);
-          const int i = i0 + 4 * u;
-          if (lane == 0 && i < ca) ts[i] -= pr[u];
-        }
+        for (int j = 0; j < 64; ++j) acc = fma(lv[j], ts[ca + j], acc);
+        ts[tid] -= acc;
       }
       __syncthreads();
     }
