@@ -287,7 +287,9 @@ def bench_batch(args):
     bt = Batch(probs)
     log(f"batch of {count}: n={bt.n} nsuper={bt.bcsc.nsuper} F={bt.bcsc.flops:.4e} "
         f"host prep {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
     f = bt.factor
+    log(f"plan and upload {time.perf_counter() - t0:.1f} s: {f.plan_stats()}")
     dvals = torch.from_numpy(bt.values(probs)).cuda()
     rhs = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, bt.n)).cuda()
     res = {}

@@ -484,16 +484,15 @@ k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __res
              const int64_t* __restrict__ gidx, const double* __restrict__ V, int64_t vld,
              double* __restrict__ X, int nrhs, int64_t ldx) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ng) return;
+  const int r = blockIdx.y;  // right-hand side
+  if (t >= ng || r >= nrhs) return;
   const int c = gcols[g0 + t];
   const int64_t e0 = gptr[c], e1 = gptr[c + 1];
-  for (int r = 0; r < nrhs; ++r) {
-    const double* v = V + (int64_t)r * vld;
-    double acc = X[(int64_t)r * ldx + c];
+  const double* v = V + (int64_t)r * vld;
+  double acc = X[(int64_t)r * ldx + c];
 #pragma unroll 8
-    for (int64_t e = e0; e < e1; ++e) acc -= v[gidx[e]];
-    X[(int64_t)r * ldx + c] = acc;
-  }
+  for (int64_t e = e0; e < e1; ++e) acc -= v[gidx[e]];
+  X[(int64_t)r * ldx + c] = acc;
 }
 
 // One forward row block of a solve panel (up to SPW = 256 columns of one
@@ -514,7 +513,8 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = sp.ncol, nch = (nc + 63) / 64;
   const double* D = L + sp.vbase;  // the panel's diagonal block, column j at D + j * ld
-  for (int r = 0; r < nrhs; ++r) {
+  {
+    const int r = blockIdx.y;  // right-hand side
     double* x = X + (int64_t)r * ldx;
     double* v = V + (int64_t)r * vld;
     for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
@@ -604,22 +604,20 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
   const int nc = sp.ncol, nr = bk.nr;
   const double* col = L + sp.vbase + bk.r0 + (int64_t)tid * sp.ld;  // column tid, block rows
   if (tid < nr) rows[tid] = Ls[sp.rows_off + bk.r0 + tid];
+  __syncthreads();
   double lv[swc::TM];
   if (tid < nc) {
 #pragma unroll
     for (int i = 0; i < swc::TM; ++i) lv[i] = i < nr ? col[i] : 0.0;
   }
+  const int r = blockIdx.y;  // right-hand side
+  if (tid < nr) xr[tid] = X[(int64_t)r * ldx + rows[tid]];
   __syncthreads();
-  for (int r = 0; r < nrhs; ++r) {
-    if (tid < nr) xr[tid] = X[(int64_t)r * ldx + rows[tid]];
-    __syncthreads();
-    if (tid < nc) {
-      double acc = 0.0;
+  if (tid < nc) {
+    double acc = 0.0;
 #pragma unroll
-      for (int i = 0; i < swc::TM; ++i) acc = fma(lv[i], i < nr ? xr[i] : 0.0, acc);
-      Pb[(int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW + tid] = acc;
-    }
-    __syncthreads();
+    for (int i = 0; i < swc::TM; ++i) acc = fma(lv[i], i < nr ? xr[i] : 0.0, acc);
+    Pb[(int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW + tid] = acc;
   }
 }
 
@@ -639,7 +637,8 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const int nc = sp.ncol, nch = (nc + 63) / 64;
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   const double* D = L + sp.vbase;
-  for (int r = 0; r < nrhs; ++r) {
+  {
+    const int r = blockIdx.y;  // right-hand side
     const double* pb = Pb + (int64_t)r * pld;
     for (int c = tid; c < nc; c += kThreads) {
       double sum = 0.0;
@@ -1277,11 +1276,11 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
     const int g0 = P.lvl_gcol[l], ng = P.lvl_gcol[l + 1] - g0;
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
     if (ng > 0)
-      hipLaunchKernelGGL(k_fwd_gather, dim3((ng + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+      hipLaunchKernelGGL(k_fwd_gather, dim3((ng + kThreads - 1) / kThreads, nrhs), dim3(kThreads), 0, st,
                          (const int*)h->dGcols, g0, ng, (const int64_t*)h->dGptr,
                          (const int64_t*)h->dGidx, (const double*)h->dV, vld, dX, nrhs, ldx);
     if (nb > 0)
-      hipLaunchKernelGGL(k_fwd_panel, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+      hipLaunchKernelGGL(k_fwd_panel, dim3(nb, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const swc::SolveBlock*)h->dFblk, b0, h->dV, vld, dX, h->dY, nrhs, ldx,
                          ldy);
   }
@@ -1290,11 +1289,11 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
     const int p0 = P.lvl_sp[l], np = P.lvl_sp[l + 1] - p0;
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
     if (nb > 0)
-      hipLaunchKernelGGL(k_bwd_partial, dim3(nb), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_bwd_partial, dim3(nb, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, (const swc::SolveBlock*)h->dFblk, b0, h->dPb,
                          pld, (const double*)dX, nrhs, ldx);
     if (np > 0)
-      hipLaunchKernelGGL(k_bwd_diag, dim3(np), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+      hipLaunchKernelGGL(k_bwd_diag, dim3(np, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
@@ -1784,7 +1783,8 @@ int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device) {
   if (int rc = check_handle(h)) return rc;
   if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_solve: not factored");
   if (h->last_info) return swc_set_error(SWC_ENOTPOS, "swc_solve: factorization failed");
-  if (!B || nrhs <= 0 || ldb < h->n) return swc_set_error(SWC_EINVAL, "swc_solve: bad argument");
+  if (!B || nrhs <= 0 || nrhs > 65535 || ldb < h->n)
+    return swc_set_error(SWC_EINVAL, "swc_solve: bad argument");
   if (int rc = ensure_work(h, nrhs)) return rc;
   double* dY = h->dRef;
   if (int rc = to_factor(h, B, on_device, ldb, dY, nrhs)) return rc;
@@ -1801,7 +1801,7 @@ int swc_refine(swc_handle* h, const double* B, double* X, int nrhs, int64_t ld, 
   if (int rc = check_handle(h)) return rc;
   if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_refine: not factored");
   if (h->last_info) return swc_set_error(SWC_ENOTPOS, "swc_refine: factorization failed");
-  if (!B || !X || nrhs <= 0 || ld < h->n || !eta)
+  if (!B || !X || nrhs <= 0 || nrhs > 65535 || ld < h->n || !eta)
     return swc_set_error(SWC_EINVAL, "swc_refine: bad argument");
   if (method < 0) method = h->factored_precision ? 1 : 0;
   if (method > 1) return swc_set_error(SWC_EINVAL, "swc_refine: method must be -1, 0 or 1");
@@ -1817,7 +1817,7 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
                  int on_device, double* eta) {
   if (int rc = check_handle(h)) return rc;
   if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_residual: no values (factor first)");
-  if (!B || !X || nrhs <= 0 || ld < h->n || !eta)
+  if (!B || !X || nrhs <= 0 || nrhs > 65535 || ld < h->n || !eta)
     return swc_set_error(SWC_EINVAL, "swc_residual: bad argument");
   if (int rc = ensure_work(h, nrhs)) return rc;
   const int n = h->n;
