@@ -311,9 +311,11 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
       const double d = bcast(y[j], j);
       const bool ok = d > 0.0;
       if (!ok && badcol == 0) badcol = j + 1;
-      const double sq = ok ? sqrt(d) : 1.0;  // keep going without NaNs; info reports it
-      if (lane == j) y[j] = sq;
-      else if (lane > j) y[j] /= sq;
+      // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column by
+      // 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
+      const double rs = ok ? rsqrt(d) : 1.0;
+      if (lane == j) y[j] = ok ? d * rs : 1.0;
+      else if (lane > j) y[j] *= rs;
 #pragma unroll
       for (int c = j + 1; c < NC; ++c)
         if (c < nc) y[c] -= y[j] * bcast(y[j], c);
@@ -390,8 +392,10 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
         const double d = bcast(y[jj], jg);
         const bool ok = d > 0.0;
         badcol = (!ok && badcol == 0) ? jg + 1 : badcol;
-        const double sq = ok ? sqrt(d) : 1.0;  // keep going without NaNs; info reports it
-        y[jj] = lane == jg ? sq : (lane > jg ? y[jj] / sq : y[jj]);
+        // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
+        // by 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
+        const double rs = ok ? rsqrt(d) : 1.0;
+        y[jj] = lane == jg ? (ok ? d * rs : 1.0) : (lane > jg ? y[jj] * rs : y[jj]);
 #pragma unroll
         for (int c = jj + 1; c < 16; ++c) y[c] -= y[jj] * bcast(y[jj], c0 + c);
       }
