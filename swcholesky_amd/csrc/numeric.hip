@@ -356,13 +356,14 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
 }
 
 // The same for panels of 33..64 columns, four waves per tile: wave w holds
-// columns [16w, 16w+16) of row `lane` (the diagonal block is padded to 64 with
-// the identity, the tile rows with zeros).  POTRF and the substitution both go
-// block column by block column: wave J factors (resp. solves) its 16 columns
-// from its registers and publishes them in LDS, then the waves to its right
-// subtract that block's products from their columns, in increasing column
-// order as before.  Per tile this is 8 dependent phases of at most 16x16
-// operations per lane instead of 64 sequential columns on one wave.
+// columns [16w, 16w+16) of row `lane` of the diagonal block (padded to 64 with
+// the identity) and of the tile's rows (padded with zeros).  POTRF and the
+// substitution go block column by block column together: in phase J wave J
+// factors its 16 columns and solves its 16 columns of the tile rows from its
+// registers and publishes both in LDS, then the waves to its right subtract
+// that block's products from their columns, in increasing column order as
+// before.  Per tile this is 4 dependent phases of at most 16x16 operations per
+// lane instead of 64 sequential columns on one wave.
 __global__ void __launch_bounds__(256)
 k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
                  const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
@@ -383,7 +384,20 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
     y[c] = (lane < nc && col < nc) ? (col <= lane ? gdiag[col * nc + lane] : 0.0)
                                    : (col == lane ? 1.0 : 0.0);
   }
+  // the tile's rows below the diagonal block (tile rows [first, nr)), loaded
+  // now so that their latency overlaps the factorization
+  const int first = tl.kind == 0 ? nc : 0;
+  const bool mine = lane >= first && lane < nr;
+  double* gtile = L + pn.vbase + tl.r0;
+  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
+  double x[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) x[c] = (mine && c0 + c < nc) ? gtile[(int64_t)(c0 + c) * pn.ld + lane] : 0.0;
   int badcol = 0;
+  // phase J: wave J factors its block column and substitutes its block of x
+  // (both from its own registers), publishes both, then the waves to its right
+  // subtract the block's products from their columns of the diagonal block
+  // and of x
   for (int J = 0; J < 4; ++J) {
     if (w == J) {
 #pragma unroll
@@ -400,7 +414,16 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
         for (int c = jj + 1; c < 16; ++c) y[c] -= y[jj] * bcast(y[jj], c0 + c);
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) sl[(c0 + c) * kLdsLd + lane] = y[c];
+      for (int jj = 0; jj < 16; ++jj) {
+        x[jj] *= 1.0 / bcast(y[jj], c0 + jj);  // L(c0+jj, c0+jj) from its lane
+#pragma unroll
+        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * bcast(y[jj], c0 + c);
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        sl[(c0 + c) * kLdsLd + lane] = y[c];
+        sx[(c0 + c) * kLdsLd + lane] = x[c];
+      }
     }
     __syncthreads();
     if (w > J) {
@@ -408,13 +431,16 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
       for (int jj = 0; jj < 16; ++jj) {
         const double* lj = sl + (16 * J + jj) * kLdsLd;
         const double lij = lj[lane];
+        const double xj = sx[(16 * J + jj) * kLdsLd + lane];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) y[c] -= lij * lj[c0 + c];
+        for (int c = 0; c < 16; ++c) {
+          const double lcj = lj[c0 + c];
+          y[c] -= lij * lcj;
+          x[c] -= xj * lcj;
+        }
       }
     }
   }
-  double* gtile = L + pn.vbase + tl.r0;
-  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
   if (tl.kind == 0) {
     if (badcol && badcol <= nc && lane == 0) atomicMin(info_out, pn.c0 + badcol);
 #pragma unroll
@@ -423,35 +449,6 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
         gtile[(int64_t)(c0 + c) * pn.ld + lane] = y[c];
         if (ftile) ftile[(int64_t)(c0 + c) * pn.ld + lane] = (float)y[c];
       }
-  }
-  // rows of this tile below the diagonal block: tile rows [first, nr)
-  const int first = tl.kind == 0 ? nc : 0;
-  const bool mine = lane >= first && lane < nr;
-  double x[16];
-#pragma unroll
-  for (int c = 0; c < 16; ++c) x[c] = (mine && c0 + c < nc) ? gtile[(int64_t)(c0 + c) * pn.ld + lane] : 0.0;
-  for (int J = 0; J < 4; ++J) {
-    if (w == J) {
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const double* lj = sl + (c0 + jj) * kLdsLd;
-        x[jj] *= 1.0 / lj[c0 + jj];
-#pragma unroll
-        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c0 + c];
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
-    }
-    __syncthreads();
-    if (w > J) {
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const double* lj = sl + (16 * J + jj) * kLdsLd;
-        const double xj = sx[(16 * J + jj) * kLdsLd + lane];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) x[c] -= xj * lj[c0 + c];
-      }
-    }
   }
 #pragma unroll
   for (int c = 0; c < 16; ++c)
