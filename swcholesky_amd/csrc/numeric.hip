@@ -588,14 +588,121 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   }
 }
 
+// Narrow solve panels (at most SPW_NARROW = 16 columns): one wave per row block
+// and per panel, everything in registers.  Forward: substitute the panel's
+// triangle (lane i holds row i), store the solution from the writer block, and
+// push L21 x of the block's rows (lane = row).
+__global__ void __launch_bounds__(64)
+k_fwd_narrow(const double* __restrict__ L, const Panel* __restrict__ spanels,
+             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
+             int64_t vld, double* __restrict__ X, double* __restrict__ Y, int64_t ldx,
+             int64_t ldy) {
+  constexpr int NW = swc::SPW_NARROW;
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
+  const Panel sp = spanels[bk.panel];
+  const int lane = threadIdx.x, r = blockIdx.y;
+  const int nc = sp.ncol;
+  double* x = X + (int64_t)r * ldx;
+  double* v = V + (int64_t)r * vld;
+  const double* D = L + sp.vbase;
+  const double* blk = D + bk.r0;
+  double row[NW], lv[NW];
+#pragma unroll
+  for (int c = 0; c < NW; ++c) {
+    row[c] = (lane < nc && c < lane) ? D[(int64_t)c * sp.ld + lane] : 0.0;
+    lv[c] = (lane < bk.nr && c < nc) ? blk[(int64_t)c * sp.ld + lane] : 0.0;
+  }
+  const double rinv = lane < nc ? 1.0 / D[(int64_t)lane * sp.ld + lane] : 1.0;
+  double ti = lane < nc ? x[sp.c0 + lane] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+    ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
+  }
+  if (bk.writer && lane < nc) Y[(int64_t)r * ldy + sp.c0 + lane] = ti;
+  if (bk.nr > 0) {
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < NW; ++c) acc = fma(lv[c], bcast(ti, c), acc);
+    if (lane < bk.nr) {
+      const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;  // position in the supernode's rows
+      if (p < sp.sw) x[sp.sc0 + p] -= acc;
+      else v[sp.voff + (p - sp.sw)] += acc;
+    }
+  }
+}
+
+// Backward row block of a narrow panel: partial[j] = sum over the block's rows
+// of L(row, j) x(row), lane = row, each column's products combined by a fixed
+// butterfly.
+__global__ void __launch_bounds__(64)
+k_bwd_partial_narrow(const double* __restrict__ L, const int* __restrict__ Ls,
+                     const Panel* __restrict__ spanels,
+                     const swc::SolveBlock* __restrict__ blocks, int b0, int bl,
+                     double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
+                     int64_t ldx) {
+  constexpr int NW = swc::SPW_NARROW;
+  const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
+  const Panel sp = spanels[bk.panel];
+  const int lane = threadIdx.x, r = blockIdx.y;
+  const int nc = sp.ncol;
+  const double* blk = L + sp.vbase + bk.r0;
+  const double xr = lane < bk.nr ? X[(int64_t)r * ldx + Ls[sp.rows_off + bk.r0 + lane]] : 0.0;
+  double a[NW];
+#pragma unroll
+  for (int c = 0; c < NW; ++c) a[c] = (lane < bk.nr && c < nc) ? blk[(int64_t)c * sp.ld + lane] * xr : 0.0;
+  double* out = Pb + (int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW;
+#pragma unroll
+  for (int c = 0; c < NW; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off);
+    if (lane == 0 && c < nc) out[c] = a[c];
+  }
+}
+
+// Backward narrow panel: t = y_p - (sum of its row blocks' partials, in block
+// order), then the transposed substitution (lane i holds column i of L11).
+__global__ void __launch_bounds__(64)
+k_bwd_diag_narrow(const double* __restrict__ L, const Panel* __restrict__ spanels,
+                  const int* __restrict__ order, int p0, const int* __restrict__ fblk_ptr,
+                  int bl, const double* __restrict__ Pb, int64_t pld,
+                  const double* __restrict__ Y, double* __restrict__ X, int64_t ldx,
+                  int64_t ldy) {
+  constexpr int NW = swc::SPW_NARROW;
+  const int qi = p0 + blockIdx.x;
+  const Panel sp = spanels[order[qi]];
+  const int lane = threadIdx.x, r = blockIdx.y;
+  const int nc = sp.ncol;
+  const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
+  const double* D = L + sp.vbase;
+  double col[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) col[j] = (lane < nc && j > lane && j < nc) ? D[(int64_t)lane * sp.ld + j] : 0.0;
+  const double rinv = lane < nc ? 1.0 / D[(int64_t)lane * sp.ld + lane] : 1.0;
+  double ti = 0.0;
+  if (lane < nc) {
+    const double* pb = Pb + (int64_t)r * pld;
+    double sum = 0.0;
+    for (int b = f0; b < f1; ++b) sum += pb[(int64_t)(b - bl) * swc::SPW + lane];
+    ti = Y[(int64_t)r * ldy + sp.c0 + lane] - sum;
+  }
+#pragma unroll
+  for (int j = NW - 1; j >= 0; --j) {
+    const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+    ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
+  }
+  if (lane < nc) X[(int64_t)r * ldx + sp.c0 + lane] = ti;
+}
+
 // Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
-// for the solve panel's columns j, into Pb at the block's slot in its level.
+// for the solve panel's columns j, into Pb at the block's slot in its level
+// (its index minus the level's first block bl).
 // Thread j reads the block's rows of column j, contiguous in memory, all at
 // once, and sums them in row order against x(rows) staged in LDS.
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ spanels, const swc::SolveBlock* __restrict__ blocks,
-              int b0, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
+              int b0, int bl, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
   __shared__ double xr[swc::TM];
   __shared__ int rows[swc::TM];
@@ -618,7 +725,7 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
     double acc = 0.0;
 #pragma unroll
     for (int i = 0; i < swc::TM; ++i) acc = fma(lv[i], i < nr ? xr[i] : 0.0, acc);
-    Pb[(int64_t)r * pld + (int64_t)blockIdx.x * swc::SPW + tid] = acc;
+    Pb[(int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW + tid] = acc;
   }
 }
 
@@ -1272,30 +1379,40 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
   const int64_t ldy = h->n;
   const Panel* sps = (const Panel*)h->dSpanels;
+  const swc::SolveBlock* fbl = (const swc::SolveBlock*)h->dFblk;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
   for (int l = 0; l < P.nslevel; ++l) {
     const int g0 = P.lvl_gcol[l], ng = P.lvl_gcol[l + 1] - g0;
-    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
+    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0, nbn = P.lvl_fblk_narrow[l];
     if (ng > 0)
       hipLaunchKernelGGL(k_fwd_gather, dim3((ng + kThreads - 1) / kThreads, nrhs), dim3(kThreads), 0, st,
                          (const int*)h->dGcols, g0, ng, (const int64_t*)h->dGptr,
                          (const int64_t*)h->dGidx, (const double*)h->dV, vld, dX, nrhs, ldx);
-    if (nb > 0)
-      hipLaunchKernelGGL(k_fwd_panel, dim3(nb, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
-                         (const swc::SolveBlock*)h->dFblk, b0, h->dV, vld, dX, h->dY, nrhs, ldx,
-                         ldy);
+    if (nbn > 0)
+      hipLaunchKernelGGL(k_fwd_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
+                         fbl, b0, h->dV, vld, dX, h->dY, ldx, ldy);
+    if (nb > nbn)
+      hipLaunchKernelGGL(k_fwd_panel, dim3(nb - nbn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
+                         sps, fbl, b0 + nbn, h->dV, vld, dX, h->dY, nrhs, ldx, ldy);
   }
   const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
   for (int l = P.nslevel - 1; l >= 0; --l) {
-    const int p0 = P.lvl_sp[l], np = P.lvl_sp[l + 1] - p0;
-    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0;
-    if (nb > 0)
-      hipLaunchKernelGGL(k_bwd_partial, dim3(nb, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, sps, (const swc::SolveBlock*)h->dFblk, b0, h->dPb,
-                         pld, (const double*)dX, nrhs, ldx);
-    if (np > 0)
-      hipLaunchKernelGGL(k_bwd_diag, dim3(np, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+    const int p0 = P.lvl_sp[l], np = P.lvl_sp[l + 1] - p0, npn = P.lvl_sp_narrow[l];
+    const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0, nbn = P.lvl_fblk_narrow[l];
+    if (nbn > 0)
+      hipLaunchKernelGGL(k_bwd_partial_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL,
+                         (const int*)h->dLs, sps, fbl, b0, b0, h->dPb, pld, (const double*)dX, ldx);
+    if (nb > nbn)
+      hipLaunchKernelGGL(k_bwd_partial, dim3(nb - nbn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
+                         (const int*)h->dLs, sps, fbl, b0 + nbn, b0, h->dPb, pld, (const double*)dX,
+                         nrhs, ldx);
+    if (npn > 0)
+      hipLaunchKernelGGL(k_bwd_diag_narrow, dim3(npn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
+                         (const double*)h->dPb, pld, (const double*)h->dY, dX, ldx, ldy);
+    if (np > npn)
+      hipLaunchKernelGGL(k_bwd_diag, dim3(np - npn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+                         (const int*)h->dSpOrder, p0 + npn, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
 }

@@ -248,15 +248,28 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     P.lvl_sp.assign(P.nslevel + 1, 0);
     for (int q = 0; q < nsp; ++q) P.lvl_sp[sp_level[q] + 1]++;
     for (int l = 0; l < P.nslevel; ++l) P.lvl_sp[l + 1] += P.lvl_sp[l];
+    // within a level, narrow solve panels (<= SPW_NARROW columns, one wave each)
+    // first, then the others
     P.sp_order.resize(nsp);
+    P.lvl_sp_narrow.assign(P.nslevel, 0);
+    for (int q = 0; q < nsp; ++q)
+      if (P.spanels[q].ncol <= SPW_NARROW) P.lvl_sp_narrow[sp_level[q]]++;
     std::vector<int> fill(P.lvl_sp.begin(), P.lvl_sp.end() - 1);
-    for (int q = 0; q < nsp; ++q) P.sp_order[fill[sp_level[q]]++] = q;
+    std::vector<int> fill_wide(P.nslevel);
+    for (int l = 0; l < P.nslevel; ++l) fill_wide[l] = P.lvl_sp[l] + P.lvl_sp_narrow[l];
+    for (int q = 0; q < nsp; ++q) {
+      const int l = sp_level[q];
+      P.sp_order[P.spanels[q].ncol <= SPW_NARROW ? fill[l]++ : fill_wide[l]++] = q;
+    }
 
     // row blocks below each solve panel's diagonal block, by level
     P.lvl_fblk.assign(P.nslevel + 1, 0);
+    P.lvl_fblk_narrow.assign(P.nslevel, 0);
     P.fblk_ptr.assign(nsp + 1, 0);
     for (int l = 0; l < P.nslevel; ++l) {
       for (int qi = P.lvl_sp[l]; qi < P.lvl_sp[l + 1]; ++qi) {
+        if (qi == P.lvl_sp[l] + P.lvl_sp_narrow[l])
+          P.lvl_fblk_narrow[l] = (int)P.fblocks.size() - P.lvl_fblk[l];
         const int q = P.sp_order[qi];
         const Panel& sp = P.spanels[q];
         P.fblk_ptr[qi] = (int)P.fblocks.size();
@@ -268,6 +281,8 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
         }
       }
       P.lvl_fblk[l + 1] = (int)P.fblocks.size();
+      if (P.lvl_sp_narrow[l] == P.lvl_sp[l + 1] - P.lvl_sp[l])
+        P.lvl_fblk_narrow[l] = P.lvl_fblk[l + 1] - P.lvl_fblk[l];
       P.max_lvl_fblk = std::max(P.max_lvl_fblk, P.lvl_fblk[l + 1] - P.lvl_fblk[l]);
     }
     P.fblk_ptr[nsp] = (int)P.fblocks.size();

@@ -19,7 +19,8 @@ namespace swc {
 
 constexpr int PW = 64;  // panel width (columns)
 constexpr int TM = 64;  // tile height (rows)
-constexpr int SPW = 256;  // solve panel width (columns)
+constexpr int SPW = 256;        // solve panel width (columns)
+constexpr int SPW_NARROW = 16;  // solve panels this narrow run one wave each
 
 struct Panel {
   int64_t vbase;  // value offset of (panel row 0, panel column 0)
@@ -97,6 +98,8 @@ struct Plan {
   std::vector<Panel> spanels;
   std::vector<int> sp_order;        // solve panels grouped by level
   std::vector<int> lvl_sp;          // nslevel+1
+  std::vector<int> lvl_sp_narrow;   // nslevel: narrow solve panels at the start of each level
+  std::vector<int> lvl_fblk_narrow; // nslevel: their row blocks at the start of the level's
   int nslevel = 0;
   // forward solve: column c (factor ordering) takes the contributions
   // V[gidx[gptr[c] .. gptr[c+1])] of its descendants, in increasing supernode
