@@ -122,39 +122,42 @@ __device__ inline unsigned landing_off(uint64_t mask, int p, int base, unsigned 
 // whose A rows are fed in the order pi(x) = x/4 + 4(x%4) so that its output
 // (D[4(l/16) + r][l%16]) lands in the f64 layout; each 16-column chunk is
 // accumulated in FP32 and then added to the FP64 accumulator.
-template <typename T>
+template <typename T, int NBJ = 4, int KSTEP = 64>
 __device__ inline void task_update(const T* __restrict__ src, int ld, int k, int ra, int ca,
                                    uint64_t rmask, uint64_t cmask, bool diag, int wave, int lane,
-                                   d4 (&acc)[4][4]) {
+                                   d4 (&acc)[4][NBJ]) {
   constexpr bool kF32 = sizeof(T) == 4;
   const int lr = lane & 15, lk = lane >> 4;
   const int arow = kF32 ? (lr >> 2) + 4 * (lr & 3) : lr;
   const unsigned kstep = (unsigned)ld * sizeof(T);  // bytes between source columns
-  unsigned ao[4], bo[4];
-  bool ab[4], bb[4];
+  unsigned ao[4], bo[NBJ];
+  bool ab[4], bb[NBJ];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     ao[b] = landing_off(rmask, 16 * b + arow, ra, sizeof(T)) + lk * kstep;
-    bo[b] = landing_off(cmask, 16 * b + lr, ca, sizeof(T)) + lk * kstep;
     ab[b] = ((rmask >> (16 * b)) & 0xffffull) != 0;
+  }
+#pragma unroll
+  for (int b = 0; b < NBJ; ++b) {
+    bo[b] = landing_off(cmask, 16 * b + lr, ca, sizeof(T)) + lk * kstep;
     bb[b] = ((cmask >> (16 * b)) & 0xffffull) != 0;
   }
-  for (int kb = 16 * wave; kb < k; kb += 64) {
+  for (int kb = 16 * wave; kb < k; kb += KSTEP) {
     const __amdgpu_buffer_rsrc_t rs = make_rsrc((const char*)src + (int64_t)kb * kstep,
                                                 (int64_t)(k - kb) * kstep);
-    T a[4][4], bv[4][4];
+    T a[4][4], bv[NBJ][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 4; ++u) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        a[b][u] = load_elem<T>(rs, ao[b] + 4 * u * kstep);
-        bv[b][u] = load_elem<T>(rs, bo[b] + 4 * u * kstep);
-      }
+      for (int b = 0; b < 4; ++b) a[b][u] = load_elem<T>(rs, ao[b] + 4 * u * kstep);
+#pragma unroll
+      for (int b = 0; b < NBJ; ++b) bv[b][u] = load_elem<T>(rs, bo[b] + 4 * u * kstep);
+    }
     if constexpr (!kF32) {
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj)
+        for (int bj = 0; bj < NBJ; ++bj)
           if (ab[bi] && bb[bj] && !(diag && bi < bj)) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -164,18 +167,18 @@ __device__ inline void task_update(const T* __restrict__ src, int ld, int k, int
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi) {
         if (!ab[bi]) continue;
-        f4 c[4];
+        f4 c[NBJ];
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) c[bj] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int bj = 0; bj < NBJ; ++bj) c[bj] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj)
+        for (int bj = 0; bj < NBJ; ++bj)
           if (bb[bj] && !(diag && bi < bj)) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
               c[bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bi][u], bv[bj][u], c[bj], 0, 0, 0);
           }
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj)
+        for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[bi][bj][r] += (double)c[bj][r];
       }
@@ -269,6 +272,58 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
       }
     }
   }
+}
+
+// Tiles of panels of at most 16 columns with a short total inner dimension: one
+// wave per tile over the whole inner dimension, a 64x16 accumulator (four 16x16
+// blocks), written back from the registers.  Same tasks, modes and write-back
+// rules as k_tile_update.
+template <int MODE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_tile_narrow(double* __restrict__ L, const Panel* __restrict__ panels,
+              const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
+              double* __restrict__ dscratch, const float* __restrict__ Lf) {
+  const Tile tl = tiles[tile0 + blockIdx.x];
+  const Panel pn = panels[tl.panel];
+  const int lane = threadIdx.x;
+  const int nr = tl.nr, nc = pn.ncol;
+  const bool diag = tl.r0 == 0;
+  double* gtile = L + pn.vbase + tl.r0;
+  d4 acc[4][1];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) acc[bi][0] = d4{0.0, 0.0, 0.0, 0.0};
+  const bool f32 = MODE == 2 || (MODE == 1 && tl.task_end > tl.task_begin &&
+                                   tasks[tl.task_begin].kind == 0);
+  if (f32) {
+    for (int t = tl.task_begin; t < tl.task_end; ++t) {
+      const Task tk = tasks[t];
+      task_update<float, 1, 16>(Lf + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask,
+                                diag, 0, lane, acc);
+    }
+  } else {
+    for (int t = tl.task_begin; t < tl.task_end; ++t) {
+      const Task tk = tasks[t];
+      task_update<double, 1, 16>(L + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask, tk.cmask,
+                                 diag, 0, lane, acc);
+    }
+  }
+  const bool copy = tl.kind == 0 && tl.slot >= 0;
+  const int j = lane & 15;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * bi + (lane >> 4) + 4 * r;
+      if (i < nr && j < nc) {
+        if (!(diag && i < j)) {
+          const double v = gtile[(int64_t)j * pn.ld + i] - acc[bi][0][r];
+          gtile[(int64_t)j * pn.ld + i] = v;
+          if (copy && i < nc) dscratch[tl.slot + j * nc + i] = v;
+        } else if (copy) {
+          dscratch[tl.slot + j * nc + i] = 0.0;
+        }
+      }
+    }
 }
 
 // Broadcast of lane src's value (src uniform over the wave).
@@ -1277,12 +1332,20 @@ void enqueue_factor(swc_handle* h, bool profile) {
     const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
     hipEvent_t* pe = profile ? &h->pev[(size_t)4 * l] : nullptr;
     if (pe) hipEventRecord(pe[0], st);
-    if (nt > 0) {
-      auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
-                                                                             : k_tile_update<2>);
-      hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, h->dL, (const Panel*)h->dPanels,
+    const int ntn = P.lvl_tiles_narrow[l];  // the level's one-wave tiles come first
+    if (ntn > 0) {
+      auto kern = h->precision == 0 ? k_tile_narrow<0> : (h->precision == 1 ? k_tile_narrow<1>
+                                                                             : k_tile_narrow<2>);
+      hipLaunchKernelGGL(kern, dim3(ntn), dim3(64), 0, st, h->dL, (const Panel*)h->dPanels,
                          (const Tile*)h->dTiles, (const Task*)h->dTasks, t0, h->dDiag,
                          (const float*)h->dLf);
+    }
+    if (nt > ntn) {
+      auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
+                                                                             : k_tile_update<2>);
+      hipLaunchKernelGGL(kern, dim3(nt - ntn), dim3(kThreads), 0, st, h->dL,
+                         (const Panel*)h->dPanels, (const Tile*)h->dTiles, (const Task*)h->dTasks,
+                         t0 + ntn, h->dDiag, (const float*)h->dLf);
     }
     if (pe) hipEventRecord(pe[1], st);
     if (ns > 0) {

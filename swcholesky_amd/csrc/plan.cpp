@@ -311,10 +311,32 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       cost[i] += (double)t.k * ((t.rb - t.ra) + (t.cb - t.ca)) + 2048.0;
   std::vector<int> order(ptiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  // Tiles of panels at most 16 wide whose tasks have a short total inner
+  // dimension (at most 256) come first in their level, where the level has at
+  // least 1024 of them: they run one wave each in their own launch.  The others
+  // keep the four-wave split of the inner dimension.
+  std::vector<char> is_narrow(ptiles.size(), 0);
+  for (size_t i = 0; i < ptiles.size(); ++i) {
+    int64_t sumk = 0;
+    for (const Task& t : ptasks[i]) sumk += t.k;
+    is_narrow[i] = P.panels[ptiles[i].t.panel].ncol <= 16 && sumk <= 256;
+  }
+  {
+    // a separate launch only pays where a level has many of them
+    std::vector<int> cnt(P.nlevel, 0);
+    for (size_t i = 0; i < ptiles.size(); ++i) cnt[ptiles[i].level] += is_narrow[i];
+    for (size_t i = 0; i < ptiles.size(); ++i)
+      if (cnt[ptiles[i].level] < 1024) is_narrow[i] = 0;
+  }
+  auto narrow = [&](int i) { return is_narrow[i] != 0; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (ptiles[a].level != ptiles[b].level) return ptiles[a].level < ptiles[b].level;
+    if (narrow(a) != narrow(b)) return narrow(a);
     return cost[a] > cost[b];
   });
+  P.lvl_tiles_narrow.assign(P.nlevel, 0);
+  for (size_t i = 0; i < ptiles.size(); ++i)
+    if (is_narrow[i]) P.lvl_tiles_narrow[ptiles[i].level]++;
   P.lvl_tiles.assign(P.nlevel + 1, 0);
   P.lvl_trsm.assign(P.nlevel + 1, 0);
   P.lvl_tile_flops.assign(P.nlevel, 0.0);

@@ -85,6 +85,8 @@ struct Plan {
   std::vector<Panel> panels;
   std::vector<Tile> tiles;          // grouped by level (one update launch per level)
   std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
+  std::vector<int> lvl_tiles_narrow;  // nlevel: one-wave tiles (panels <= 16 wide, short
+                                      // total inner dimension), first in their level
   std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)
   std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles
   std::vector<int> lvl_trsm_cls;    // 4 per level: starts of the panel-width classes
