@@ -1,7 +1,7 @@
 """Numpy emulation of the GPU kernels on the real device plan (test infrastructure).
 
-Runs the same index arithmetic as k_tile_update / k_panel_factor / k_fwd_diag /
-k_fwd_update / k_bwd_partial / k_bwd_diag in swcholesky_amd/csrc/numeric.hip, asserting that every
+Runs the same index arithmetic as the tile update, panel factor and solve
+kernels of swcholesky_amd/csrc/numeric.hip, asserting that every
 index stays inside its buffer, so plan and kernel indexing are checked on the CPU.
 """
 import numpy as np
