@@ -1762,6 +1762,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
     return swc_set_error(SWC_EINVAL, "swc_upload: blockSet must span 0..n");
   h->factored = false;
   h->drop_graphs();
+  h->work_cols = 0;  // the solve buffers are sized by the structure: rebuilt on next use
   h->n = n;
   h->nsuper = supNo;
   h->super.assign(blockSet, blockSet + supNo + 1);

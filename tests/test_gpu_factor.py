@@ -120,3 +120,21 @@ def test_graph_replays_match_eager_launches():
         f.set_precision(mode)
         assert f.factor(prob.values) == 0
     assert np.array_equal(f.L_values(), L_eager)
+
+
+def test_reupload_larger_structure_into_one_handle():
+    """A handle that solved a small structure is reused for a larger one: the
+    solve buffers sized by the structure must follow it."""
+    small, large = SMALL[0][1](), SMALL[3][1]()
+    bs = S.analyze_p2(small.n, small.colptr, small.rowidx, small.perm)
+    bl = S.analyze_p2(large.n, large.colptr, large.rowidx, large.perm)
+    f = S.Factor(bs)
+    assert f.factor(small.values) == 0
+    f.solve(np.ones(small.n))
+    S._lib.check(S._lib.lib().swc_upload_symbolic(f._h, bl._h))
+    f.bcsc, f.n = bl, bl.n
+    assert f.factor(large.values) == 0
+    A, rp, ci, v = full_csr(large)
+    rhs = A @ np.random.default_rng(3).uniform(-1, 1, large.n)
+    x = f.solve(rhs)
+    assert O.backward_error(rp, ci, v, x, rhs) < 1e-14
