@@ -1755,8 +1755,9 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
                const int* Ls, const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                const int* A2p, const int* A2i, const int64_t* a2src, int64_t nvals) {
   if (int rc = check_handle(h)) return rc;
-  if (n <= 0 || supNo <= 0 || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr || !pruneSet ||
-      !A2p || !A2i)
+  // pruneSet may be null when it is empty (no supernode has descendants)
+  if (n <= 0 || supNo <= 0 || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr ||
+      (!pruneSet && prunePtr[supNo] != 0) || !A2p || !A2i)
     return swc_set_error(SWC_EINVAL, "swc_upload: bad argument");
   if (blockSet[0] != 0 || blockSet[supNo] != n)
     return swc_set_error(SWC_EINVAL, "swc_upload: blockSet must span 0..n");

@@ -72,7 +72,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     }
   }
 
-  auto describe = [&](int q, const Src& sr, Task& t) {
+  auto describe = [&](int q, const Src& sr, Task& t, int nsrc = 1) {
     const Panel& pn = P.panels[q];
     if (sr.d >= 0) {
       const int d = sr.d;
@@ -81,12 +81,15 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       t.k = super[d + 1] - super[d];
       t.rows_off = (int)pi[d];
     } else {
+      // panels [-(sr.d + 1), -(sr.d + 1) + nsrc) of the same supernode: their
+      // columns are consecutive in the supernode's column-major block
       const Panel& sp = P.panels[-(sr.d + 1)];
       const int s = pn.sn;
       const int qoff = sp.c0 - super[s];
       t.src_base = px[s] + (int64_t)qoff * pn.ld;
       t.ld = pn.ld;
-      t.k = sp.ncol;
+      t.k = 0;
+      for (int i = 0; i < nsrc; ++i) t.k += P.panels[-(sr.d + 1) + i].ncol;
       t.rows_off = (int)pi[s];
     }
     t.ca = sr.ca;
@@ -156,14 +159,15 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       ptasks[ti].push_back(t);
     }
   };
-  // task of tile (panel q, rows r0..) from the earlier panel q2 of the same supernode
-  auto add_panel_task = [&](int ti, int q2) {
+  // task of tile (panel q, rows r0..) from the earlier panels q2 .. q2+nsrc-1
+  // of the same supernode (one inner dimension over all their columns)
+  auto add_panel_task = [&](int ti, int q2, int nsrc) {
     const Tile& tl = ptiles[ti].t;
     const int q = tl.panel;
     const Panel& pn = P.panels[q];
     const int off = pn.c0 - super[pn.sn];
     Task t;
-    describe(q, {-(q2 + 1), off, off + pn.ncol}, t);
+    describe(q, {-(q2 + 1), off, off + pn.ncol}, t, nsrc);
     t.ra = off + tl.r0;
     t.rb = t.ra + tl.nr;
     ptasks[ti].push_back(t);
@@ -184,16 +188,25 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       }
     }
     P.panel_level[fp] = le;
-    // right-looking steps: step p at level le+p+1 updates panels J > p from panel p
+    // right-looking steps over blocks of RLB panels: step p at level le+p+1
+    // updates the later panels of p's block from panel p; the block's last step
+    // updates every panel after the block from all of the block's panels at
+    // once (inner dimension up to RLB * PW).  Panel p+1 is updated at step p
+    // either way, and is factored at that level.
     for (int p = 0; p + 1 < np; ++p) {
       const int lvl = le + p + 1;
-      for (int J = p + 1; J < np; ++J) {
+      const int b0 = p - p % RLB, b1 = std::min(b0 + RLB, np);
+      const bool last = p + 1 == b1;
+      for (int J = p + 1; J < (last ? np : b1); ++J) {
         const Panel& pn = P.panels[fp + J];
         for (int r0 = 0; r0 < pn.nrow; r0 += TM) {
           const bool diag = r0 == 0;
           const int kind = (diag && J == p + 1) ? 0 : 1;
           const int ti = add_tile(fp + J, r0, kind, lvl, !diag && J == p + 1);
-          add_panel_task(ti, fp + p);
+          if (last)
+            add_panel_task(ti, fp + b0, b1 - b0);
+          else
+            add_panel_task(ti, fp + p, 1);
         }
       }
       P.panel_level[fp + p + 1] = lvl;
@@ -448,7 +461,9 @@ extern "C" {
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out) {
-  if (!out || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr || !pruneSet || supNo <= 0)
+  // pruneSet may be null when it is empty (no supernode has descendants)
+  if (!out || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr || supNo <= 0 ||
+      (!pruneSet && prunePtr[supNo] != 0))
     return swc_set_error(SWC_EINVAL, "swc_plan_build: bad argument");
   *out = nullptr;
   try {

@@ -9,8 +9,9 @@
 // tile then factors the panel, and the other tiles are solved against it.
 // A wider supernode first applies all descendant updates to every tile of its
 // lower trapezoid in a single level (fully 2D-parallel), then factors itself
-// right-looking: step p factors panel p and updates the trailing tiles from it,
-// one level per step.
+// right-looking, one level per panel, over blocks of RLB panels: within a block
+// each factored panel updates the block's later panels; after the block's last
+// panel the block updates all later panels with one wide inner dimension.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -19,6 +20,7 @@ namespace swc {
 
 constexpr int PW = 64;  // panel width (columns)
 constexpr int TM = 64;  // tile height (rows)
+constexpr int RLB = 8;  // panels per block of a wide supernode's right-looking steps
 constexpr int SPW = 256;        // solve panel width (columns)
 constexpr int SPW_NARROW = 16;  // solve panels this narrow run one wave each
 

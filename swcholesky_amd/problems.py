@@ -24,7 +24,7 @@ import scipy.sparse as sp
 
 __all__ = [
     "Problem", "laplacian2d_const", "laplacian2d_var", "laplacian3d_var",
-    "vector3d_var", "gmrf2d", "grid_nd_order", "write_mtx", "read_mtx",
+    "vector3d_var", "gmrf2d", "dense_spd", "grid_nd_order", "write_mtx", "read_mtx",
     "write_perm", "read_perm", "lower_csc_from_coo", "full_csr_from_lower",
 ]
 
@@ -176,6 +176,18 @@ def vector3d_var(side=48, seed=2, wlo=1e-2, whi=1e2, shift=1e-6):
     cp, ri, va = lower_csc_from_coo(n, rows[keep], cols[keep], vals[keep])
     return Problem(n, cp, ri, va, grid_nd_order(shape, dof=3), f"vec3d_var_{side}",
                    shape, dof=3)
+
+
+def dense_spd(n=1100, seed=3):
+    """Dense SPD matrix M M^T / n + I (M uniform in [-1, 1)) in the identity
+    ordering: a single supernode of n columns, factored entirely by the
+    right-looking steps of a wide supernode."""
+    rng = np.random.default_rng(seed)
+    m = rng.uniform(-1.0, 1.0, (n, n))
+    a = m @ m.T / n + np.eye(n)
+    r, c = np.tril_indices(n)
+    cp, ri, va = lower_csc_from_coo(n, r, c, a[r, c])
+    return Problem(n, cp, ri, va, np.arange(n), f"dense_{n}")
 
 
 def gmrf2d(side=256, seed=4, tau=1.0, kappa2=0.05, wlo=0.5, whi=2.0):

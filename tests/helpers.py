@@ -46,4 +46,5 @@ SMALL = [
     ("lap3d_var_20", lambda: P.laplacian3d_var(20)),
     ("vec3d_var_6", lambda: P.vector3d_var(6)),
     ("gmrf2d_32", lambda: P.gmrf2d(32)),
+    ("dense_1100", lambda: P.dense_spd(1100)),
 ]

@@ -21,7 +21,8 @@ def _amap(b):
 
 @pytest.mark.parametrize("make", [lambda: P.laplacian2d_const(10), lambda: P.laplacian2d_var(37, 3),
                                   lambda: P.laplacian3d_var(12), lambda: P.vector3d_var(6),
-                                  lambda: P.laplacian2d_const(100), lambda: P.vector3d_var(9)])
+                                  lambda: P.laplacian2d_const(100), lambda: P.vector3d_var(9),
+                                  lambda: P.dense_spd(1100)])
 def test_emulated_kernels_match_oracle(make):
     prob = make()
     b, a2x, Lo, _ = setup(prob)
@@ -37,25 +38,67 @@ def test_emulated_kernels_match_oracle(make):
     assert relF(xe, xo) < 1e-6
 
 
-def test_plan_covers_every_update_once():
-    """Each (target entry, source) update appears in exactly one task."""
-    prob = P.laplacian3d_var(16)
+def _expected_sources(b):
+    """For every pair (r, c), r >= c, of rows of L: the number of columns m < c
+    whose structure holds both r and c, and the sums of m and m^2 over them,
+    leaving out the columns of c's own panel (the panel factorization applies
+    those)."""
+    n = b.n
+    cnt, s1, s2 = (np.zeros((n, n)) for _ in range(3))
+    for d in range(b.nsuper):
+        R = b.s[b.pi[d]:b.pi[d + 1]]
+        W = b.super[d + 1] - b.super[d]
+        i = np.arange(len(R))
+        # columns of d before each row's column, not counting that column's panel
+        m = np.where(i < W, 64 * (i // 64), W)
+        m0 = b.super[d]
+        ir, ic = np.tril_indices(len(R))
+        r, c, k = R[ir], R[ic], m[ic]
+        cnt[r, c] += k
+        s1[r, c] += k * m0 + k * (k - 1) / 2
+        s2[r, c] += sum((m0 + j) ** 2 * (k > j) for j in range(W))
+    return cnt, s1, s2
+
+
+@pytest.mark.parametrize("make", [lambda: P.vector3d_var(9), lambda: P.laplacian3d_var(12),
+                                  lambda: P.dense_spd(1100)])
+def test_plan_covers_every_update_once(make):
+    """Every source column m of every update L(r,c) -= L(r,m) L(c,m) is applied
+    by exactly one task: per target pair, the count of source columns and the
+    sums of m and m^2 over the tasks' inner dimensions equal the structural
+    ones.  The dense case is one supernode of 18 panels, whose right-looking
+    steps apply blocks of panels as one source."""
+    prob = make()
     b, *_ = setup(prob)
     plan = api.build_plan(b)
     panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
-    seen = {}
+    n = b.n
+    cnt, s1, s2 = (np.zeros((n, n)) for _ in range(3))
+    wide = 0
     for tl in tiles:
         pn = panels[tl["panel"]]
         for t in tasks[tl["task_begin"]:tl["task_end"]]:
             rows = b.s[t["rows_off"] + t["ra"]:t["rows_off"] + t["rb"]]
             cols = b.s[t["rows_off"] + t["ca"]:t["rows_off"] + t["cb"]]
-            key = (int(t["src_base"]), int(t["k"]))
-            for r in rows:
-                for c in cols:
-                    if r >= c:
-                        seen.setdefault((int(r), int(c), key), 0)
-                        seen[(int(r), int(c), key)] += 1
-    assert all(v == 1 for v in seen.values())
+            assert np.all(cols >= pn["c0"]) and np.all(cols < pn["c0"] + pn["ncol"])
+            # first global column of the source: the supernode whose rows the task indexes
+            d = int(np.searchsorted(b.pi, t["rows_off"], side="right") - 1)
+            assert b.pi[d] == t["rows_off"]
+            ld = b.pi[d + 1] - b.pi[d]
+            m0 = b.super[d] + (int(t["src_base"]) - b.p[b.super[d]]) // ld
+            k = int(t["k"])
+            assert m0 + k <= b.super[d + 1]
+            wide += k > 64 and d == pn["sn"]  # a block of panels of the same supernode
+            rr, cc = np.meshgrid(rows, cols, indexing="ij")
+            keep = rr >= cc
+            r, c = rr[keep], cc[keep]
+            np.add.at(cnt, (r, c), k)
+            np.add.at(s1, (r, c), k * m0 + k * (k - 1) / 2)
+            np.add.at(s2, (r, c), sum((m0 + j) ** 2 for j in range(k)))
+    ec, e1, e2 = _expected_sources(b)
+    assert np.array_equal(cnt, ec) and np.array_equal(s1, e1) and np.array_equal(s2, e2)
+    if b.maxSupWid > 16 * 64:
+        assert wide > 0
     # the forward-solve panel order is sorted by level
     lp = plan["lvl_panels"]
     assert lp[0] == 0 and lp[-1] == len(panels) and np.all(np.diff(lp) >= 0)
