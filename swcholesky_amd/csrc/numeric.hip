@@ -323,128 +323,62 @@ __device__ inline double bcast(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
-// Panel factorization, one wave per tile of a panel completed at this level,
-// for panels of at most NC (16 or 32) columns: POTRF of the panel's nc x nc
-// diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430) in registers --
-// lane i holds row i, the pivot and the multipliers of column j are read from
-// their lanes -- then B <- B L11^{-T} for the tile's rows below the diagonal
-// block (dtrsm 'R','L','T','N', .c:461-467), lane i substituting its own row.
-// Both apply the updates of an entry in increasing column order, as the
-// left-looking loops of the reference do.  Every tile of a panel computes the
-// same L11 from the same launch-invariant copy (bitwise identical), so no tile
-// waits for another; the diagonal tile stores it and reports info.  Entries
-// above the diagonal of the register copy are never read by lanes on or below
-// it, so they are left unmasked.
-template <int NC>
-__global__ void __launch_bounds__(64)
+// Panel factorization of the panels completed at this level, one workgroup of
+// NW waves per tile, for panels of at most 16 NW columns: POTRF of the panel's
+// nc x nc diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430), then
+// B <- B L11^{-T} for the tile's rows below the diagonal block (dtrsm
+// 'R','L','T','N', .c:461-467).  Every tile of a panel computes the same L11
+// from the same launch-invariant copy (bitwise identical), so no tile waits
+// for another; the diagonal tile stores it and reports info.
+//
+// Wave w holds columns [16w, 16w+16) of row `lane` of the diagonal block
+// (padded to 16 NW with the identity) and of the tile's rows (padded with
+// zeros).  Both go block column by block column: in phase J wave J factors its
+// 16 columns, publishing each column of L11 in LDS as it completes; the other
+// rows of the block column read its entries from there (broadcast reads), and
+// only the entry the next pivot depends on is read from its lane.  Wave J then
+// substitutes its 16 columns of the tile rows (scaling by the pivots'
+// reciprocal square roots) and publishes them; the waves to its right
+// subtract the block's products from their columns of both, in increasing
+// column order, as the left-looking loops of the reference do.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
                const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
                const double* __restrict__ dscratch, int* __restrict__ info_out,
                float* __restrict__ Lf) {
-  const Tile tl = tiles[list[list0 + blockIdx.x]];
-  const Panel pn = panels[tl.panel];
-  const int lane = threadIdx.x;
-  const int nr = tl.nr, nc = pn.ncol;
-  const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
-  double y[NC];                              // row `lane` of the diagonal block
-#pragma unroll
-  for (int c = 0; c < NC; ++c) y[c] = (lane < nc && c < nc && c <= lane) ? gdiag[c * nc + lane] : 0.0;
-  int badcol = 0;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    if (j < nc) {
-      const double d = bcast(y[j], j);
-      const bool ok = d > 0.0;
-      if (!ok && badcol == 0) badcol = j + 1;
-      // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column by
-      // 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
-      const double rs = ok ? rsqrt(d) : 1.0;
-      if (lane == j) y[j] = ok ? d * rs : 1.0;
-      else if (lane > j) y[j] *= rs;
-#pragma unroll
-      for (int c = j + 1; c < NC; ++c)
-        if (c < nc) y[c] -= y[j] * bcast(y[j], c);
-    }
-  }
-  double* gtile = L + pn.vbase + tl.r0;
-  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
-  if (tl.kind == 0) {
-    if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-      if (lane < nc && c <= lane) {
-        gtile[(int64_t)c * pn.ld + lane] = y[c];
-        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)y[c];
-      }
-  }
-  // rows of this tile below the diagonal block: tile rows [first, nr)
-  const int first = tl.kind == 0 ? nc : 0;
-  const bool mine = lane >= first && lane < nr;
-  double x[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    if (j < nc) {
-      x[j] *= 1.0 / bcast(y[j], j);
-#pragma unroll
-      for (int c = j + 1; c < NC; ++c)
-        if (c < nc) x[c] -= x[j] * bcast(y[j], c);
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
-    if (mine && c < nc) {
-      gtile[(int64_t)c * pn.ld + lane] = x[c];
-      if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
-    }
-}
-
-// The same for panels of 33..64 columns, four waves per tile: wave w holds
-// columns [16w, 16w+16) of row `lane` of the diagonal block (padded to 64 with
-// the identity) and of the tile's rows (padded with zeros).  POTRF and the
-// substitution go block column by block column together: in phase J wave J
-// factors its 16 columns and solves its 16 columns of the tile rows from its
-// registers and publishes both in LDS, then the waves to its right subtract
-// that block's products from their columns, in increasing column order as
-// before.  Per tile this is 4 dependent phases of at most 16x16 operations per
-// lane instead of 64 sequential columns on one wave.
-__global__ void __launch_bounds__(256)
-k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
-                 const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
-                 const double* __restrict__ dscratch, int* __restrict__ info_out,
-                 float* __restrict__ Lf) {
-  __shared__ double sl[64 * kLdsLd];  // L11: column c at sl[c * kLdsLd]
-  __shared__ double sx[64 * kLdsLd];  // solved tile columns, same layout
+  constexpr int NCOL = 16 * NW;
+  __shared__ double sl[NCOL * kLdsLd];  // L11: column c at sl[c * kLdsLd]
+  __shared__ double sx[NCOL * kLdsLd];  // solved tile columns, same layout
+  __shared__ double sinv[NCOL];         // 1 / L11(c, c)
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nr = tl.nr, nc = pn.ncol;
   const int c0 = 16 * w;  // first column of this wave
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
-  double y[16];
-#pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    const int col = c0 + c;
-    y[c] = (lane < nc && col < nc) ? (col <= lane ? gdiag[col * nc + lane] : 0.0)
-                                   : (col == lane ? 1.0 : 0.0);
-  }
-  // the tile's rows below the diagonal block (tile rows [first, nr)), loaded
-  // now so that their latency overlaps the factorization
+  // the tile's rows below the diagonal block are tile rows [first, nr)
   const int first = tl.kind == 0 ? nc : 0;
   const bool mine = lane >= first && lane < nr;
   double* gtile = L + pn.vbase + tl.r0;
   float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
-  double x[16];
+  // every lane loads (entries outside the block or the tile read element 0,
+  // which both have), then selects: no branches around the loads, so the
+  // tile's latency overlaps the diagonal block's
+  double y[16], x[16];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) x[c] = (mine && c0 + c < nc) ? gtile[(int64_t)(c0 + c) * pn.ld + lane] : 0.0;
+  for (int c = 0; c < 16; ++c) {
+    const int col = c0 + c;
+    const bool iny = lane < nc && col < nc && col <= lane, inx = mine && col < nc;
+    const double vy = gdiag[iny ? col * nc + lane : 0];
+    const double vx = gtile[inx ? (int64_t)col * pn.ld + lane : 0];
+    y[c] = iny ? vy : ((lane >= nc || col >= nc) && col == lane ? 1.0 : 0.0);
+    x[c] = inx ? vx : 0.0;
+  }
   int badcol = 0;
-  // phase J: wave J factors its block column and substitutes its block of x
-  // (both from its own registers), publishes both, then the waves to its right
-  // subtract the block's products from their columns of the diagonal block
-  // and of x
-  for (int J = 0; J < 4; ++J) {
+  for (int J = 0; J < NW; ++J) {
     if (w == J) {
+      double* colj = sl + c0 * kLdsLd;  // column c0 + jj at colj[jj * kLdsLd]
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const int jg = c0 + jj;
@@ -455,22 +389,25 @@ k_panel_factor64(double* __restrict__ L, const Panel* __restrict__ panels,
         // by 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
         const double rs = ok ? rsqrt(d) : 1.0;
         y[jj] = lane == jg ? (ok ? d * rs : 1.0) : (lane > jg ? y[jj] * rs : y[jj]);
+        if (lane == jg) sinv[jg] = rs;
+        colj[jj * kLdsLd + lane] = y[jj];
+        if (jj < 15) {
+          y[jj + 1] -= y[jj] * bcast(y[jj], jg + 1);
 #pragma unroll
-        for (int c = jj + 1; c < 16; ++c) y[c] -= y[jj] * bcast(y[jj], c0 + c);
+          for (int c = jj + 2; c < 16; ++c) y[c] -= y[jj] * colj[jj * kLdsLd + c0 + c];
+        }
       }
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
-        x[jj] *= 1.0 / bcast(y[jj], c0 + jj);  // L(c0+jj, c0+jj) from its lane
+        x[jj] *= sinv[c0 + jj];
+        const double* lj = colj + jj * kLdsLd + c0;
 #pragma unroll
-        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * bcast(y[jj], c0 + c);
+        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c];
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        sl[(c0 + c) * kLdsLd + lane] = y[c];
-        sx[(c0 + c) * kLdsLd + lane] = x[c];
-      }
+      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
     }
-    __syncthreads();
+    if (NW > 1) __syncthreads();
     if (w > J) {
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
@@ -1338,13 +1275,13 @@ void enqueue_factor(swc_handle* h, bool profile) {
       const Tile* tp = (const Tile*)h->dTiles;
       const int* lp = (const int*)h->dTrsm;
       if (cls[1] > cls[0])
-        hipLaunchKernelGGL(k_panel_factor<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
+        hipLaunchKernelGGL(k_panel_factor<1>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
                            lp, cls[0], (const double*)h->dDiag, h->dInfo, lf);
       if (cls[2] > cls[1])
-        hipLaunchKernelGGL(k_panel_factor<32>, dim3(cls[2] - cls[1]), dim3(64), 0, st, h->dL, pp, tp,
+        hipLaunchKernelGGL(k_panel_factor<2>, dim3(cls[2] - cls[1]), dim3(128), 0, st, h->dL, pp, tp,
                            lp, cls[1], (const double*)h->dDiag, h->dInfo, lf);
       if (cls[3] > cls[2])
-        hipLaunchKernelGGL(k_panel_factor64, dim3(cls[3] - cls[2]), dim3(kThreads), 0, st, h->dL, pp,
+        hipLaunchKernelGGL(k_panel_factor<4>, dim3(cls[3] - cls[2]), dim3(256), 0, st, h->dL, pp,
                            tp, lp, cls[2], (const double*)h->dDiag, h->dInfo, lf);
     }
     if (pe) (void)hipEventRecord(pe[2], st);
