@@ -6,7 +6,7 @@
 // panel DAG is one launch of the tile kernel (every update of every tile of
 // the level) followed by the panel factor launches (POTRF of the diagonal
 // block of each panel the level completes and TRSM of its tiles).  The solves
-// walk solve panels of <= 256 columns, level by level, forward then backward.
+// walk solve panels of <= SPW columns, level by level, forward then backward.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -323,6 +323,86 @@ __device__ inline double bcast(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
+// Panel factorization for panels of at most 16 columns (NC), one wave per tile
+// and registers only, for levels with many narrow panels (the leaves, batched
+// instances), where how many workgroups fit on a CU matters more than one
+// tile's latency: POTRF of the panel's nc x nc
+// diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430) in registers --
+// lane i holds row i, the pivot and the multipliers of column j are read from
+// their lanes -- then B <- B L11^{-T} for the tile's rows below the diagonal
+// block (dtrsm 'R','L','T','N', .c:461-467), lane i substituting its own row.
+// Both apply the updates of an entry in increasing column order, as the
+// left-looking loops of the reference do.  Every tile of a panel computes the
+// same L11 from the same launch-invariant copy (bitwise identical), so no tile
+// waits for another; the diagonal tile stores it and reports info.  Entries
+// above the diagonal of the register copy are never read by lanes on or below
+// it, so they are left unmasked.
+template <int NC>
+__global__ void __launch_bounds__(64)
+k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
+               const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
+               const double* __restrict__ dscratch, int* __restrict__ info_out,
+               float* __restrict__ Lf) {
+  const Tile tl = tiles[list[list0 + blockIdx.x]];
+  const Panel pn = panels[tl.panel];
+  const int lane = threadIdx.x;
+  const int nr = tl.nr, nc = pn.ncol;
+  const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
+  double y[NC];                              // row `lane` of the diagonal block
+#pragma unroll
+  for (int c = 0; c < NC; ++c) y[c] = (lane < nc && c < nc && c <= lane) ? gdiag[c * nc + lane] : 0.0;
+  int badcol = 0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (j < nc) {
+      const double d = bcast(y[j], j);
+      const bool ok = d > 0.0;
+      if (!ok && badcol == 0) badcol = j + 1;
+      // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column by
+      // 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
+      const double rs = ok ? rsqrt(d) : 1.0;
+      if (lane == j) y[j] = ok ? d * rs : 1.0;
+      else if (lane > j) y[j] *= rs;
+#pragma unroll
+      for (int c = j + 1; c < NC; ++c)
+        if (c < nc) y[c] -= y[j] * bcast(y[j], c);
+    }
+  }
+  double* gtile = L + pn.vbase + tl.r0;
+  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
+  if (tl.kind == 0) {
+    if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (lane < nc && c <= lane) {
+        gtile[(int64_t)c * pn.ld + lane] = y[c];
+        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)y[c];
+      }
+  }
+  // rows of this tile below the diagonal block: tile rows [first, nr)
+  const int first = tl.kind == 0 ? nc : 0;
+  const bool mine = lane >= first && lane < nr;
+  double x[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (j < nc) {
+      x[j] *= 1.0 / bcast(y[j], j);
+#pragma unroll
+      for (int c = j + 1; c < NC; ++c)
+        if (c < nc) x[c] -= x[j] * bcast(y[j], c);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (mine && c < nc) {
+      gtile[(int64_t)c * pn.ld + lane] = x[c];
+      if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+    }
+}
+
+
 // Panel factorization of the panels completed at this level, one workgroup of
 // NW waves per tile, for panels of at most 16 NW columns: POTRF of the panel's
 // nc x nc diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430), then
@@ -349,7 +429,10 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
                float* __restrict__ Lf) {
   constexpr int NCOL = 16 * NW;
   __shared__ double sl[NCOL * kLdsLd];  // L11: column c at sl[c * kLdsLd]
-  __shared__ double sx[NCOL * kLdsLd];  // solved tile columns, same layout
+  // solved tile columns, same layout (read by the waves to the right only:
+  // one wave needs none, and the smaller LDS footprint keeps more workgroups
+  // of the narrow class resident)
+  __shared__ double sx[NW > 1 ? NCOL * kLdsLd : 1];
   __shared__ double sinv[NCOL];         // 1 / L11(c, c)
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
@@ -405,10 +488,10 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
         for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c];
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
+      for (int c = 0; c < 16 && NW > 1; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
     }
     if (NW > 1) __syncthreads();
-    if (w > J) {
+    if (NW > 1 && w > J) {
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const double* lj = sl + (16 * J + jj) * kLdsLd;
@@ -470,7 +553,7 @@ k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __res
   X[(int64_t)r * ldx + c] = acc;
 }
 
-// One forward row block of a solve panel (up to SPW = 256 columns of one
+// One forward row block of a solve panel (up to SPW = 128 columns of one
 // supernode): solve the panel's diagonal block (every block of the panel, from
 // the same inputs) in 64-column chunks -- wave 0 substitutes the chunk's 64x64
 // triangle from registers, then all threads subtract its products from the
@@ -1275,7 +1358,7 @@ void enqueue_factor(swc_handle* h, bool profile) {
       const Tile* tp = (const Tile*)h->dTiles;
       const int* lp = (const int*)h->dTrsm;
       if (cls[1] > cls[0])
-        hipLaunchKernelGGL(k_panel_factor<1>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
+        hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
                            lp, cls[0], (const double*)h->dDiag, h->dInfo, lf);
       if (cls[2] > cls[1])
         hipLaunchKernelGGL(k_panel_factor<2>, dim3(cls[2] - cls[1]), dim3(128), 0, st, h->dL, pp, tp,

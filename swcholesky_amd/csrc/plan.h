@@ -21,7 +21,7 @@ namespace swc {
 constexpr int PW = 64;  // panel width (columns)
 constexpr int TM = 64;  // tile height (rows)
 constexpr int RLB = 8;  // panels per block of a wide supernode's right-looking steps
-constexpr int SPW = 256;        // solve panel width (columns)
+constexpr int SPW = 128;       // solve panel width (columns)
 constexpr int SPW_NARROW = 16;  // solve panels this narrow run one wave each
 
 struct Panel {

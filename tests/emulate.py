@@ -97,7 +97,7 @@ def factor(plan, Ls, xsize, amap, vals):
 def solve(plan, Ls, L, b):
     """Forward then backward solve in the factor ordering (b is an n-vector),
     as k_fwd_gather / k_fwd_panel / k_bwd_partial / k_bwd_diag run it over the
-    solve panels (up to 256 columns of one supernode): the forward sweep
+    solve panels (up to SPW columns of one supernode): the forward sweep
     accumulates right-hand sides in x and writes the panel solutions to y; the
     backward sweep reads y and the final values of x, and overwrites x."""
     x = np.array(b, dtype=np.float64)

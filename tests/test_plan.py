@@ -106,7 +106,7 @@ def test_plan_covers_every_update_once(make):
 
 @pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(20), lambda: P.vector3d_var(10)])
 def test_emulated_solve_wide_supernodes(make):
-    """Solve panels of several 256-column chunks (supernodes wider than 256)."""
+    """Supernodes wider than a solve panel (SPW columns) solved in several panels."""
     prob = make()
     b, a2x, Lo, _ = setup(prob)
     assert b.maxSupWid > 256
