@@ -553,12 +553,38 @@ k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __res
   X[(int64_t)r * ldx + c] = acc;
 }
 
+// Substitution of one 64-column chunk, lane i holding row i: x_j = t_j / L_jj,
+// then t_i -= L_ij x_j for the rows below (forward, row[j] = L_ij), or for the
+// rows above in decreasing j (backward, row[j] = L_ji, the transposed chunk).
+template <bool FORWARD>
+__device__ inline double substitute64(const double (&row)[64], double rinv, double ti, int lane) {
+  if constexpr (FORWARD) {
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+      const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
+      ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
+    }
+  } else {
+#pragma unroll
+    for (int j = 63; j >= 0; --j) {
+      const double xj = bcast(ti * rinv, j);
+      ti = lane == j ? xj : (lane < j ? ti - row[j] * xj : ti);
+    }
+  }
+  return ti;
+}
+
 // One forward row block of a solve panel (up to SPW = 128 columns of one
-// supernode): solve the panel's diagonal block (every block of the panel, from
-// the same inputs) in 64-column chunks -- wave 0 substitutes the chunk's 64x64
-// triangle from registers, then all threads subtract its products from the
-// panel's later rows -- write the solution if this is the panel's writer
-// block, then push L21 x of the block's rows.
+// supernode, two 64-column chunks): solve the panel's diagonal block (every
+// block of the panel, from the same inputs), write the solution if this is the
+// panel's writer block, then push L21 x of the block's rows.  Each wave loads
+// what its step needs at the start, so all loads are in flight at once: wave 0
+// the first chunk's triangle (lane = row), wave 1 the second chunk's rows of the
+// first chunk's columns, wave 2 the second chunk's triangle, and every wave its
+// share of the block's rows.  Then wave 0 substitutes the first chunk, wave 1
+// subtracts its products from the second chunk's entries, wave 2 substitutes
+// the second chunk.
+static_assert(swc::SPW == 128, "the solve kernels split a solve panel in two 64-column chunks");
 __global__ void __launch_bounds__(kThreads)
 k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
@@ -569,79 +595,75 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
   const Panel sp = spanels[bk.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = sp.ncol, nch = (nc + 63) / 64;
+  const int nc = sp.ncol, na0 = min(64, nc), na1 = nc - 64;  // chunk widths (na1 <= 0: one chunk)
   const double* D = L + sp.vbase;  // the panel's diagonal block, column j at D + j * ld
-  {
-    const int r = blockIdx.y;  // right-hand side
-    double* x = X + (int64_t)r * ldx;
-    double* v = V + (int64_t)r * vld;
-    for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
-    __syncthreads();
-    for (int a = 0; a < nch; ++a) {
-      const int ca = 64 * a, na = min(64, nc - ca);
-      if (wave == 0) {
-        // row `lane` of the chunk's triangle (zero beyond na) and the
-        // reciprocal of its diagonal entry (1 beyond na: padded unknowns stay 0)
-        double row[64];
+  const int r = blockIdx.y;        // right-hand side
+  double* x = X + (int64_t)r * ldx;
+  double* v = V + (int64_t)r * vld;
+  // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
+  constexpr int Q = swc::SPW / 4;
+  const double* blk = L + sp.vbase + bk.r0;
+  double bv[Q];
 #pragma unroll
-        for (int c = 0; c < 64; ++c)
-          row[c] = (lane < na && c < lane) ? D[(int64_t)(ca + c) * sp.ld + ca + lane] : 0.0;
-        const double rinv = lane < na ? 1.0 / D[(int64_t)(ca + lane) * sp.ld + ca + lane] : 1.0;
-        double ti = lane < na ? xs[ca + lane] : 0.0;
-        // column-oriented substitution: x_j = t_j / L_jj, then t_i -= L_ij x_j
+  for (int q = 0; q < Q; ++q) {
+    const int j = wave + 4 * q;
+    const bool in = lane < bk.nr && j < nc;
+    const double val = blk[in ? (int64_t)j * sp.ld + lane : 0];
+    bv[q] = in ? val : 0.0;
+  }
+  // this wave's step: element (row0 + lane, col0 + c), c < 64, where valid
+  const int row0 = wave == 0 ? 0 : 64, col0 = wave == 2 ? 64 : 0;
+  const int nrow = wave == 0 ? na0 : na1, ncol = wave == 2 ? na1 : na0;
+  double m[64];
 #pragma unroll
-        for (int j = 0; j < 64; ++j) {
-          const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-          ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
-        }
-        if (lane < na) xs[ca + lane] = ti;
-      }
-      __syncthreads();
-      const int i = ca + 64 + tid;  // a later row of the diagonal block
-      if (i < nc) {
-        double acc = 0.0;
-        for (int c0 = 0; c0 < na; c0 += 32) {  // 32 loads in flight per step (coalesced over i)
-          double lv[32];
-#pragma unroll
-          for (int u = 0; u < 32; ++u) lv[u] = c0 + u < na ? D[(int64_t)(ca + c0 + u) * sp.ld + i] : 0.0;
-#pragma unroll
-          for (int u = 0; u < 32; ++u) acc = fma(lv[u], xs[ca + c0 + u], acc);
-        }
-        xs[i] -= acc;
-      }
-      __syncthreads();
-    }
-    if (bk.writer)
-      for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
-    if (bk.nr > 0) {
-      // wave w sums columns j = w, w+4, ... for row `lane`
-      const double* blk = L + sp.vbase + bk.r0;
+  for (int c = 0; c < 64; ++c) {
+    const bool in = wave < 3 && lane < nrow && c < ncol && (wave == 1 || c < lane);
+    const double val = D[in ? (int64_t)(col0 + c) * sp.ld + row0 + lane : 0];
+    m[c] = in ? val : 0.0;
+  }
+  const bool dg = (wave == 0 || wave == 2) && lane < nrow;
+  const double dval = D[dg ? (int64_t)(row0 + lane) * sp.ld + row0 + lane : 0];
+  // reciprocal of the diagonal entry; 1 beyond the chunk (padded unknowns stay 0)
+  const double rinv = dg ? 1.0 / dval : 1.0;
+  for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
+  __syncthreads();
+  if (wave == 0) {
+    const double ti = substitute64<true>(m, rinv, lane < na0 ? xs[lane] : 0.0, lane);
+    if (lane < na0) xs[lane] = ti;
+  }
+  __syncthreads();
+  if (na1 > 0) {
+    if (wave == 1 && lane < na1) {
       double acc = 0.0;
-      if (lane < bk.nr) {
-        for (int q0 = 0; wave + 4 * q0 < nc; q0 += 16) {  // 16 loads in flight per step
-          double lv[16];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) {
-            const int j = wave + 4 * (q0 + u);
-            lv[u] = j < nc ? blk[(int64_t)j * sp.ld + lane] : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 16; ++u) {
-            const int j = wave + 4 * (q0 + u);
-            acc = fma(lv[u], j < nc ? xs[j] : 0.0, acc);
-          }
-        }
-      }
-      part[wave][lane] = acc;
-      __syncthreads();
-      if (wave == 0 && lane < bk.nr) {
-        const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-        const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;  // position in the supernode's rows
-        if (p < sp.sw) x[sp.sc0 + p] -= y;               // a later column of the supernode
-        else v[sp.voff + (p - sp.sw)] += y;              // a row of an ancestor
-      }
+      for (int c = 0; c < 64; ++c) acc = fma(m[c], xs[c], acc);  // na0 = 64 here
+      xs[64 + lane] -= acc;
     }
     __syncthreads();
+    if (wave == 2) {
+      const double ti = substitute64<true>(m, rinv, lane < na1 ? xs[64 + lane] : 0.0, lane);
+      if (lane < na1) xs[64 + lane] = ti;
+    }
+    __syncthreads();
+  }
+  if (bk.writer)
+    for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
+  if (bk.nr > 0) {
+    // wave w sums columns j = w, w+4, ... for row `lane`, in increasing j
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = wave + 4 * q;
+      acc = fma(bv[q], j < nc ? xs[j] : 0.0, acc);
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && lane < bk.nr) {
+      const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;  // position in the supernode's rows
+      if (p < sp.sw) x[sp.sc0 + p] -= y;               // a later column of the supernode
+      else v[sp.voff + (p - sp.sw)] += y;              // a row of an ancestor
+    }
   }
 }
 
@@ -787,9 +809,12 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
 }
 
 // Backward solve panel: t = y_p - (sum of its row blocks' partials, in block
-// order), then L11^T x_p = t in 64-column chunks from the last: wave 0
-// substitutes the chunk's transposed triangle from registers, then all threads
-// subtract its products from the panel's earlier entries.
+// order), then L11^T x_p = t chunk by chunk from the last.  As in the forward
+// kernel each wave loads its step's data at the start: wave 0 the second
+// chunk's transposed triangle (lane = column), wave 1 the second chunk's rows
+// of the first chunk's columns (lane = column), wave 2 the first chunk's
+// transposed triangle.  Wave 0 substitutes the second chunk, wave 1 subtracts
+// its products from the first chunk's entries, wave 2 substitutes the first.
 __global__ void __launch_bounds__(kThreads)
 k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
            const int* __restrict__ order, int p0, const int* __restrict__ fblk_ptr, int b0,
@@ -799,60 +824,57 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const int qi = p0 + blockIdx.x;
   const Panel sp = spanels[order[qi]];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = sp.ncol, nch = (nc + 63) / 64;
+  const int nc = sp.ncol, na0 = min(64, nc), na1 = nc - 64;  // chunk widths (na1 <= 0: one chunk)
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   const double* D = L + sp.vbase;
-  {
-    const int r = blockIdx.y;  // right-hand side
-    const double* pb = Pb + (int64_t)r * pld;
-    for (int c = tid; c < nc; c += kThreads) {
-      double sum = 0.0;
-      for (int b = f0; b < f1; b += 8) {  // 8 loads in flight per step, summed in order
-        double pv[8];
+  const int r = blockIdx.y;  // right-hand side
+  // this wave's step: m[j] = L(rowc + j, colc + lane) where valid
+  const int colc = wave == 0 ? 64 : 0, rowc = wave == 2 ? 0 : 64;
+  const int ncl = wave == 0 ? na1 : na0, nrw = wave == 2 ? na0 : na1;
+  double m[64];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) pv[u] = b + u < f1 ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+  for (int j = 0; j < 64; ++j) {
+    const bool in = wave < 3 && lane < ncl && j < nrw && (wave == 1 || j > lane);
+    const double val = D[in ? (int64_t)(colc + lane) * sp.ld + rowc + j : 0];
+    m[j] = in ? val : 0.0;
+  }
+  const bool dg = (wave == 0 || wave == 2) && lane < ncl;
+  const double dval = D[dg ? (int64_t)(colc + lane) * sp.ld + colc + lane : 0];
+  const double rinv = dg ? 1.0 / dval : 1.0;
+  const double* pb = Pb + (int64_t)r * pld;
+  for (int c = tid; c < nc; c += kThreads) {
+    double sum = 0.0;
+    for (int b = f0; b < f1; b += 8) {  // 8 loads in flight per step, summed in order
+      double pv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) sum += pv[u];
-      }
-      ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - sum;
+      for (int u = 0; u < 8; ++u) pv[u] = b + u < f1 ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sum += pv[u];
+    }
+    ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - sum;
+  }
+  __syncthreads();
+  if (na1 > 0) {
+    if (wave == 0) {
+      const double ti = substitute64<false>(m, rinv, lane < na1 ? ts[64 + lane] : 0.0, lane);
+      if (lane < na1) ts[64 + lane] = ti;
     }
     __syncthreads();
-    for (int a = nch - 1; a >= 0; --a) {
-      const int ca = 64 * a, na = min(64, nc - ca);
-      if (wave == 0) {
-        // column `lane` of the chunk's triangle below the diagonal (zero beyond
-        // na) and the reciprocal of its diagonal entry
-        double col[64];
+    // first-chunk entries: t_i -= sum_j L(64 + j, i) x_(64 + j)
+    if (wave == 1) {
+      double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 64; ++j)
-          col[j] = (lane < na && j > lane && j < na) ? D[(int64_t)(ca + lane) * sp.ld + ca + j] : 0.0;
-        const double rinv = lane < na ? 1.0 / D[(int64_t)(ca + lane) * sp.ld + ca + lane] : 1.0;
-        double ti = lane < na ? ts[ca + lane] : 0.0;
-        // x_j = t_j / L_jj, then t_i -= L_ji x_j for i < j
-#pragma unroll
-        for (int j = 63; j >= 0; --j) {
-          const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-          ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
-        }
-        if (lane < na) ts[ca + lane] = ti;
-      }
-      __syncthreads();
-      // earlier entries: t_i -= sum_j L(ca + j, i) x_(ca + j) for i < ca; thread
-      // i reads the chunk's rows of column i, contiguous in memory, all at once
-      if (tid < ca) {
-        double lv[64];
-#pragma unroll
-        for (int j = 0; j < 64; ++j) lv[j] = j < na ? D[(int64_t)tid * sp.ld + ca + j] : 0.0;
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 64; ++j) acc = fma(lv[j], ts[ca + j], acc);
-        ts[tid] -= acc;
-      }
-      __syncthreads();
+      for (int j = 0; j < 64; ++j) acc = fma(m[j], j < na1 ? ts[64 + j] : 0.0, acc);
+      ts[lane] -= acc;  // na0 = 64 here
     }
-    for (int c = tid; c < nc; c += kThreads) X[(int64_t)r * ldx + sp.c0 + c] = ts[c];
     __syncthreads();
   }
+  if (wave == 2) {
+    const double ti = substitute64<false>(m, rinv, lane < na0 ? ts[lane] : 0.0, lane);
+    if (lane < na0) ts[lane] = ti;
+  }
+  __syncthreads();
+  for (int c = tid; c < nc; c += kThreads) X[(int64_t)r * ldx + sp.c0 + c] = ts[c];
 }
 
 // y[k] = b[perm[k]] (to the factor's ordering) or x[perm[k]] = y[k] (back).
