@@ -601,14 +601,15 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   double* x = X + (int64_t)r * ldx;
   double* v = V + (int64_t)r * vld;
   // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
+  // (entries outside the block read the panel's first diagonal entry, which
+  // exists even when the block is empty, and are replaced by zero)
   constexpr int Q = swc::SPW / 4;
-  const double* blk = L + sp.vbase + bk.r0;
   double bv[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = wave + 4 * q;
     const bool in = lane < bk.nr && j < nc;
-    const double val = blk[in ? (int64_t)j * sp.ld + lane : 0];
+    const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
     bv[q] = in ? val : 0.0;
   }
   // this wave's step: element (row0 + lane, col0 + c), c < 64, where valid
@@ -776,36 +777,43 @@ k_bwd_diag_narrow(const double* __restrict__ L, const Panel* __restrict__ spanel
 // Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
 // for the solve panel's columns j, into Pb at the block's slot in its level
 // (its index minus the level's first block bl).
-// Thread j reads the block's rows of column j, contiguous in memory, all at
-// once, and sums them in row order against x(rows) staged in LDS.
+// Thread t takes column t mod SPW and half t / SPW of the block's rows, reads
+// them (contiguous in memory) all at once, and sums them in row order against
+// x(rows) staged in LDS; the two halves are then added, first half first.
+static_assert(kThreads == 2 * swc::SPW && swc::TM % 2 == 0, "two threads per column");
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ spanels, const swc::SolveBlock* __restrict__ blocks,
               int b0, int bl, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
+  constexpr int H = swc::TM / 2;
   __shared__ double xr[swc::TM];
-  __shared__ int rows[swc::TM];
+  __shared__ double part[2][swc::SPW];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
   const Panel sp = spanels[bk.panel];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, c = tid % swc::SPW, h = tid / swc::SPW;
   const int nc = sp.ncol, nr = bk.nr;
-  const double* col = L + sp.vbase + bk.r0 + (int64_t)tid * sp.ld;  // column tid, block rows
-  if (tid < nr) rows[tid] = Ls[sp.rows_off + bk.r0 + tid];
-  __syncthreads();
-  double lv[swc::TM];
-  if (tid < nc) {
-#pragma unroll
-    for (int i = 0; i < swc::TM; ++i) lv[i] = i < nr ? col[i] : 0.0;
-  }
   const int r = blockIdx.y;  // right-hand side
-  if (tid < nr) xr[tid] = X[(int64_t)r * ldx + rows[tid]];
-  __syncthreads();
-  if (tid < nc) {
-    double acc = 0.0;
+  // column c, rows [H h, H h + H) of the block (entries outside it read the
+  // panel's first diagonal entry, which always exists, and are replaced by zero)
+  const double* D = L + sp.vbase;
+  const int64_t off = (int64_t)c * sp.ld + bk.r0 + H * h;
+  double lv[H];
 #pragma unroll
-    for (int i = 0; i < swc::TM; ++i) acc = fma(lv[i], i < nr ? xr[i] : 0.0, acc);
-    Pb[(int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW + tid] = acc;
+  for (int i = 0; i < H; ++i) {
+    const bool in = c < nc && H * h + i < nr;
+    const double val = D[in ? off + i : 0];
+    lv[i] = in ? val : 0.0;
   }
+  if (tid < nr) xr[tid] = X[(int64_t)r * ldx + Ls[sp.rows_off + bk.r0 + tid]];
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) acc = fma(lv[i], H * h + i < nr ? xr[H * h + i] : 0.0, acc);
+  part[h][c] = acc;
+  __syncthreads();
+  if (h == 0 && c < nc)
+    Pb[(int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW + c] = part[0][c] + part[1][c];
 }
 
 // Backward solve panel: t = y_p - (sum of its row blocks' partials, in block
