@@ -849,17 +849,27 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const bool dg = (wave == 0 || wave == 2) && lane < ncl;
   const double dval = D[dg ? (int64_t)(colc + lane) * sp.ld + colc + lane : 0];
   const double rinv = dg ? 1.0 / dval : 1.0;
+  // t = y - (sum of the row blocks' partials): thread t sums half t / SPW of
+  // the blocks for column t mod SPW, in block order, 16 loads in flight per
+  // step; the halves are added first half first
   const double* pb = Pb + (int64_t)r * pld;
-  for (int c = tid; c < nc; c += kThreads) {
+  {
+    const int c = tid % swc::SPW, h = tid / swc::SPW;
+    const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
     double sum = 0.0;
-    for (int b = f0; b < f1; b += 8) {  // 8 loads in flight per step, summed in order
-      double pv[8];
+    if (c < nc) {
+      for (int b = fa; b < fb; b += 16) {
+        double pv[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) pv[u] = b + u < f1 ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+        for (int u = 0; u < 16; ++u)
+          pv[u] = b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) sum += pv[u];
+        for (int u = 0; u < 16; ++u) sum += pv[u];
+      }
     }
-    ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - sum;
+    if (h == 1) ts[c] = sum;
+    __syncthreads();
+    if (h == 0 && c < nc) ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - (sum + ts[c]);
   }
   __syncthreads();
   if (na1 > 0) {
