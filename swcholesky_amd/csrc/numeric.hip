@@ -427,12 +427,10 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
                const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
                const double* __restrict__ dscratch, int* __restrict__ info_out,
                float* __restrict__ Lf) {
+  static_assert(NW >= 2, "panels of at most 16 columns use k_panel_factor_regs");
   constexpr int NCOL = 16 * NW;
   __shared__ double sl[NCOL * kLdsLd];  // L11: column c at sl[c * kLdsLd]
-  // solved tile columns, same layout (read by the waves to the right only:
-  // one wave needs none, and the smaller LDS footprint keeps more workgroups
-  // of the narrow class resident)
-  __shared__ double sx[NW > 1 ? NCOL * kLdsLd : 1];
+  __shared__ double sx[NCOL * kLdsLd];  // solved tile columns, same layout
   __shared__ double sinv[NCOL];         // 1 / L11(c, c)
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
@@ -488,10 +486,10 @@ k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
         for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c];
       }
 #pragma unroll
-      for (int c = 0; c < 16 && NW > 1; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
+      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
     }
-    if (NW > 1) __syncthreads();
-    if (NW > 1 && w > J) {
+    __syncthreads();
+    if (w > J) {
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const double* lj = sl + (16 * J + jj) * kLdsLd;
