@@ -40,6 +40,15 @@ typedef struct swc_symbolic swc_symbolic;
 
 int swc_analyze(int n, const int* Ap, const int* Ai, const int64_t* perm, const int* nrelax,
                 const double* zrelax, swc_symbolic** out);
+/* Batches (no counterpart in the reference, which analyses one matrix per
+ * process): count independent instances (instance k: n[k] columns, lower
+ * triangle Ap[k]/Ai[k], ordering perm[k]) analysed on nthreads host threads (0:
+ * all cores) and concatenated into the symbolic factor of their block-diagonal
+ * operator, equal to swc_analyze of that operator.  A2SRC indexes the
+ * instances' input entries concatenated in instance order. */
+int swc_analyze_batch(int count, const int* n, const int* const* Ap, const int* const* Ai,
+                      const int64_t* const* perm, const int* nrelax, const double* zrelax,
+                      int nthreads, swc_symbolic** out);
 void swc_symbolic_free(swc_symbolic* sym);
 /* sizes[8] = {n, nsuper, ssize, xsize, maxSupWid, maxCol, nnz(tril A), |pruneSet|};
  * *flops = sum_j ColCount[j]^2 (ColumnCount.h:489-499). */

@@ -21,6 +21,8 @@ c_ip = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "swc_last_error": (ctypes.c_char_p, []),
     "swc_analyze": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    "swc_analyze_batch": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int,
+                                  ctypes.POINTER(c_vp)]),
     "swc_symbolic_free": (None, [c_vp]),
     "swc_symbolic_sizes": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_symbolic_copy": (c_int, [c_vp, c_int, c_vp]),

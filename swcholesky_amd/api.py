@@ -56,6 +56,31 @@ class BCSC:
         return np.ascontiguousarray(np.asarray(values, dtype=np.float64)[self.a2src])
 
 
+def analyze_batch(instances, nrelax=(4, 16, 48), zrelax=(0.8, 0.1, 0.05), threads=0):
+    """Symbolic analysis of independent instances [(n, colptr, rowidx, perm), ...]
+    on `threads` host threads (0: all cores), concatenated into the analysis of
+    their block-diagonal operator (equal to analyze_p2 of that operator)."""
+    arrs = [(int(n), np.ascontiguousarray(c, dtype=np.int32), np.ascontiguousarray(r, dtype=np.int32),
+             np.ascontiguousarray(p, dtype=np.int64)) for n, c, r, p in instances]
+    count = len(arrs)
+    ns = np.array([a[0] for a in arrs], dtype=np.int32)
+    vp = ctypes.c_void_p * count
+    aps = vp(*[a[1].ctypes.data for a in arrs])
+    ais = vp(*[a[2].ctypes.data for a in arrs])
+    pps = vp(*[a[3].ctypes.data for a in arrs])
+    nr = np.asarray(nrelax, dtype=np.int32)
+    zr = np.asarray(zrelax, dtype=np.float64)
+    h = ctypes.c_void_p()
+    lib = _lib.lib()
+    check(lib.swc_analyze_batch(count, ptr(ns), ctypes.cast(aps, ctypes.c_void_p),
+                                ctypes.cast(ais, ctypes.c_void_p), ctypes.cast(pps, ctypes.c_void_p),
+                                ptr(nr), ptr(zr), int(threads), ctypes.byref(h)))
+    sizes = np.zeros(8, dtype=np.int64)
+    fl = ctypes.c_double()
+    check(lib.swc_symbolic_sizes(h, ptr(sizes), ctypes.byref(fl)))
+    return BCSC(h, sizes, fl.value)
+
+
 def analyze_p2(n, colptr, rowidx, perm, nrelax=(4, 16, 48), zrelax=(0.8, 0.1, 0.05)):
     """Symbolic analysis for a GIVEN ordering (perm[new] = old)."""
     colptr = np.ascontiguousarray(colptr, dtype=np.int32)

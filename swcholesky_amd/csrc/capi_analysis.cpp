@@ -1,8 +1,12 @@
 // C ABI of the host symbolic analysis (see include/swcholesky_amd.h).
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <exception>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "analysis.h"
 #include "capi_common.h"
@@ -28,6 +32,116 @@ int swc_analyze(int n, const int* Ap, const int* Ai, const int64_t* perm, const 
     return swc_set_error(SWC_ENOMEM, "swc_analyze: out of host memory");
   } catch (const std::exception& e) {
     return swc_set_error(SWC_EINVAL, std::string("swc_analyze: ") + e.what());
+  }
+}
+
+// Independent instances analysed one by one on nthreads host threads, then
+// concatenated as the analysis of the block-diagonal operator they form:
+// instance k's columns, supernodes, row lists, values, descendant sets and
+// input entries follow those of instances 0..k-1, with every index offset by
+// theirs.  The result equals swc_analyze on the block-diagonal matrix (its
+// elimination forest has one tree per instance, and the postorder keeps each
+// tree's columns in place).
+int swc_analyze_batch(int count, const int* n, const int* const* Ap, const int* const* Ai,
+                      const int64_t* const* perm, const int* nrelax, const double* zrelax,
+                      int nthreads, swc_symbolic** out) {
+  if (!out || count <= 0 || !n || !Ap || !Ai || !perm)
+    return swc_set_error(SWC_EINVAL, "swc_analyze_batch: bad argument");
+  *out = nullptr;
+  for (int k = 0; k < count; ++k)
+    if (n[k] <= 0 || !Ap[k] || !Ai[k] || !perm[k])
+      return swc_set_error(SWC_EINVAL, "swc_analyze_batch: bad instance " + std::to_string(k));
+  static const int kNrelax[3] = {4, 16, 48};
+  static const double kZrelax[3] = {0.8, 0.1, 0.05};
+  const int* nr = nrelax ? nrelax : kNrelax;
+  const double* zr = zrelax ? zrelax : kZrelax;
+  try {
+    std::vector<swc::Symbolic> parts(count);
+    std::vector<std::exception_ptr> err(count);
+    std::atomic<int> next(0);
+    auto work = [&]() {
+      for (int k = next++; k < count; k = next++) {
+        try {
+          swc::analyze(n[k], Ap[k], Ai[k], perm[k], nr, zr, parts[k]);
+        } catch (...) {
+          err[k] = std::current_exception();
+        }
+      }
+    };
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min(nt, count);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (int k = 0; k < count; ++k)
+      if (err[k]) std::rethrow_exception(err[k]);
+
+    auto* h = new swc_symbolic;
+    swc::Symbolic& S = h->S;
+    int64_t N = 0, NS = 0, SS = 0, X = 0, NNZ = 0, NP = 0;
+    for (const auto& P : parts) {
+      N += P.n; NS += P.nsuper; SS += P.ssize; X += P.xsize;
+      NNZ += (int64_t)P.a2i.size(); NP += (int64_t)P.pruneset.size();
+    }
+    if (N > 0x7fffffff || NS > 0x7fffffff || NNZ > 0x7fffffff || NP > 0x7fffffff) {
+      delete h;
+      return swc_set_error(SWC_EINVAL, "swc_analyze_batch: batch too large for 32-bit indices");
+    }
+    S.n = (int)N;
+    S.nsuper = (int)NS;
+    S.ssize = SS;
+    S.xsize = X;
+    S.perm.reserve(N); S.colcount.reserve(N); S.parent.reserve(N); S.col2sup.reserve(N);
+    S.super.reserve(NS + 1); S.sparent.reserve(NS); S.pi.reserve(NS + 1); S.s.reserve(SS);
+    S.i_ptr.reserve(N + 1); S.p.reserve(N + 1); S.pruneptr.reserve(NS + 1); S.pruneset.reserve(NP);
+    S.a2p.reserve(N + 1); S.a2i.reserve(NNZ); S.a2src.reserve(NNZ);
+    int c0 = 0, s0 = 0;
+    int64_t r0 = 0, x0 = 0, e0 = 0, q0 = 0, in0 = 0;  // rows, values, a2 entries, prune, input
+    for (int k = 0; k < count; ++k) {
+      const swc::Symbolic& P = parts[k];
+      auto sh = [](int v, int o) { return v < 0 ? v : v + o; };
+      for (int j = 0; j < P.n; ++j) {
+        S.perm.push_back(P.perm[j] + c0);
+        S.colcount.push_back(P.colcount[j]);
+        S.parent.push_back(sh(P.parent[j], c0));
+        S.col2sup.push_back(P.col2sup[j] + s0);
+        S.i_ptr.push_back(P.i_ptr[j] + r0);
+        S.p.push_back(P.p[j] + x0);
+        S.a2p.push_back((int)(P.a2p[j] + e0));
+      }
+      for (int s = 0; s < P.nsuper; ++s) {
+        S.super.push_back(P.super[s] + c0);
+        S.sparent.push_back(sh(P.sparent[s], s0));
+        S.pi.push_back(P.pi[s] + r0);
+        S.pruneptr.push_back((int)(P.pruneptr[s] + q0));
+      }
+      for (int v : P.s) S.s.push_back(v + c0);
+      for (int v : P.pruneset) S.pruneset.push_back(v + s0);
+      for (int v : P.a2i) S.a2i.push_back(v + c0);
+      for (int64_t v : P.a2src) S.a2src.push_back(v + in0);
+      S.max_sup_wid = std::max(S.max_sup_wid, P.max_sup_wid);
+      S.max_col = std::max(S.max_col, P.max_col);
+      S.flops += P.flops;
+      c0 += P.n; s0 += P.nsuper; r0 += P.ssize; x0 += P.xsize;
+      e0 += (int64_t)P.a2i.size(); q0 += (int64_t)P.pruneset.size();
+      in0 += (int64_t)Ap[k][P.n];
+    }
+    // closing entries; p[n] follows the reference's convention for the last
+    // supernode (p[n-1] + its row count), as the last instance has it
+    const swc::Symbolic& last = parts[count - 1];
+    S.super.push_back(c0);
+    S.pi.push_back(r0);
+    S.pruneptr.push_back((int)q0);
+    S.i_ptr.push_back(r0);
+    S.p.push_back(last.p[last.n] + (x0 - last.xsize));
+    S.a2p.push_back((int)e0);
+    *out = h;
+    return SWC_OK;
+  } catch (const std::bad_alloc&) {
+    return swc_set_error(SWC_ENOMEM, "swc_analyze_batch: out of host memory");
+  } catch (const std::exception& e) {
+    return swc_set_error(SWC_EINVAL, std::string("swc_analyze_batch: ") + e.what());
   }
 }
 

@@ -34,3 +34,18 @@ def test_batch_structure_equals_instances():
     assert sn == B.nsuper
     assert np.isclose(B.flops, sum(S.analyze_p2(p.n, p.colptr, p.rowidx, p.perm).flops
                                    for p in probs))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 0])
+def test_threaded_batch_analysis_equals_blockdiag_analysis(threads):
+    """The instances analysed independently on host threads and concatenated give,
+    array for array, the analysis of the block-diagonal operator."""
+    probs = _instances(9, seed=5, lo=8, hi=30)
+    bt = Batch(probs, threads=threads)
+    ref = S.analyze_p2(bt.n, bt.colptr, bt.rowidx, bt.perm)
+    got = bt.bcsc
+    for f in ("n", "nsuper", "ssize", "xsize", "maxSupWid", "maxCol", "nnz", "nprune", "flops"):
+        assert getattr(got, f) == getattr(ref, f), f
+    for f in ("Perm", "ColCount", "parent", "super", "sParent", "col2Sup", "pi", "s", "i_ptr",
+              "p", "prunePtr", "pruneSet", "a2p", "a2i", "a2src"):
+        assert np.array_equal(getattr(got, f), getattr(ref, f)), f
