@@ -598,6 +598,46 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const int r = blockIdx.y;        // right-hand side
   double* x = X + (int64_t)r * ldx;
   double* v = V + (int64_t)r * vld;
+  if (nc <= 64) {
+    // one chunk: wave 0 substitutes it, then the block's rows are pushed
+    for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
+    __syncthreads();
+    if (wave == 0) {
+      double row[64];
+#pragma unroll
+      for (int c = 0; c < 64; ++c) {
+        const bool in = lane < nc && c < lane;
+        const double val = D[in ? (int64_t)c * sp.ld + lane : 0];
+        row[c] = in ? val : 0.0;
+      }
+      const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
+      const double rinv = lane < nc ? 1.0 / dval : 1.0;
+      const double ti = substitute64<true>(row, rinv, lane < nc ? xs[lane] : 0.0, lane);
+      if (lane < nc) xs[lane] = ti;
+    }
+    __syncthreads();
+    if (bk.writer)
+      for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
+    if (bk.nr > 0) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int j = wave + 4 * q;
+        const bool in = lane < bk.nr && j < nc;
+        const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
+        acc = fma(in ? val : 0.0, j < nc ? xs[j] : 0.0, acc);
+      }
+      part[wave][lane] = acc;
+      __syncthreads();
+      if (wave == 0 && lane < bk.nr) {
+        const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+        const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;
+        if (p < sp.sw) x[sp.sc0 + p] -= y;
+        else v[sp.voff + (p - sp.sw)] += y;
+      }
+    }
+    return;
+  }
   // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
   // (entries outside the block read the panel's first diagonal entry, which
   // exists even when the block is empty, and are replaced by zero)
@@ -834,6 +874,47 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   const double* D = L + sp.vbase;
   const int r = blockIdx.y;  // right-hand side
+  if (nc <= 64) {
+    // one chunk: wave 0 loads its transposed triangle while the partials are
+    // summed, then substitutes it
+    double col[64];
+    double rinv = 1.0;
+    if (wave == 0) {
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const bool in = lane < nc && j > lane && j < nc;
+        const double val = D[in ? (int64_t)lane * sp.ld + j : 0];
+        col[j] = in ? val : 0.0;
+      }
+      const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
+      rinv = lane < nc ? 1.0 / dval : 1.0;
+    }
+    const double* pb = Pb + (int64_t)r * pld;
+    {
+      const int c = tid % swc::SPW, h = tid / swc::SPW;
+      const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
+      double sum = 0.0;
+      if (c < nc) {
+        for (int b = fa; b < fb; b += 16) {
+          double pv[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            pv[u] = b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) sum += pv[u];
+        }
+      }
+      if (h == 1) ts[c] = sum;
+      __syncthreads();
+      if (h == 0 && c < nc) ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - (sum + ts[c]);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const double ti = substitute64<false>(col, rinv, lane < nc ? ts[lane] : 0.0, lane);
+      if (lane < nc) X[(int64_t)r * ldx + sp.c0 + lane] = ti;
+    }
+    return;
+  }
   // this wave's step: m[j] = L(rowc + j, colc + lane) where valid
   const int colc = wave == 0 ? 64 : 0, rowc = wave == 2 ? 0 : 64;
   const int ncl = wave == 0 ? na1 : na0, nrw = wave == 2 ? na0 : na1;
