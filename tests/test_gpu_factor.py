@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 import swcholesky_amd as S
+from swcholesky_amd import problems as P
 import oracle_py as O
 from helpers import SMALL, full_csr, relF, setup
 
@@ -138,3 +139,24 @@ def test_reupload_larger_structure_into_one_handle():
     rhs = A @ np.random.default_rng(3).uniform(-1, 1, large.n)
     x = f.solve(rhs)
     assert O.backward_error(rp, ci, v, x, rhs) < 1e-14
+
+
+@pytest.mark.parametrize("n", [17, 64, 65, 129, 192, 193, 300])
+def test_single_supernode_widths(n):
+    """A dense SPD matrix is one supernode of n columns: its panels and solve panels
+    take every width class (one 64-column chunk, two chunks with a one-column second
+    chunk, a trailing narrow panel, ...).  Factor and two right-hand sides against the
+    oracle."""
+    prob = P.dense_spd(n, seed=n)
+    b, a2x, Lo, info = setup(prob)
+    assert info == 0 and b.nsuper == 1
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    assert relF(f.L_values(), Lo) < 1e-12
+    A, rp, ci, v = full_csr(prob)
+    X0 = np.random.default_rng(n).uniform(-1, 1, (n, 2))
+    B = A @ X0
+    X = f.solve(B)
+    for k in range(2):
+        assert O.backward_error(rp, ci, v, X[:, k], B[:, k]) < 1e-14
+    assert np.linalg.norm(X - X0) / np.linalg.norm(X0) < 1e-10
