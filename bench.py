@@ -74,8 +74,8 @@ def main():
                          "copy kernel of known size, to scale HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library factors each matrix on one GPU "
+              "and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -19,7 +19,8 @@ HOST_SOURCES = ["analysis.cpp", "capi_analysis.cpp", "capi_common.cpp"]
 HIP_SOURCES = ["numeric.hip"]
 HOST_SOURCES += ["plan.cpp"]
 
-# -packed-fp32-ops: keep hipcc from forming v_pk_*_f32 ("Build").
+# -packed-fp32-ops: keep hipcc from forming packed FP32 instructions (v_pk_*_f32),
+# some forms of which have given wrong results on this hardware.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

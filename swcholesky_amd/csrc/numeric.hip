@@ -551,56 +551,93 @@ k_fwd_gather(const int* __restrict__ gcols, int g0, int ng, const int64_t* __res
   X[(int64_t)r * ldx + c] = acc;
 }
 
-// Substitution of one 64-column chunk, lane i holding row i: x_j = t_j / L_jj,
-// then t_i -= L_ij x_j for the rows below (forward, row[j] = L_ij), or for the
-// rows above in decreasing j (backward, row[j] = L_ji, the transposed chunk).
-template <bool FORWARD>
-__device__ inline double substitute64(const double (&row)[64], double rinv, double ti, int lane) {
-  if constexpr (FORWARD) {
+// Substitution of one 64-column chunk for RB right-hand sides at once, lane i
+// holding row i of each: x_j = t_j / L_jj, then t_i -= L_ij x_j (forward,
+// row[j] = L_ij, j increasing) or t_i -= L_ji x_j (backward, row[j] = L_ji, the
+// transposed chunk, j decreasing).  row[j] must be zero wherever lane i does
+// not depend on x_j (j >= i forward, j <= i backward, and rows outside the
+// chunk), so every lane applies every step unmasked and its own entry is left
+// as t_i after the steps it depends on: x_i = t_i / L_ii at the end.  The RB
+// chains are independent and interleaved step by step, so their latencies
+// overlap.
+template <bool FORWARD, int RB>
+__device__ inline void substitute64(const double (&row)[64], double rinv, double (&ti)[RB]) {
 #pragma unroll
-    for (int j = 0; j < 64; ++j) {
-      const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-      ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
-    }
-  } else {
+  for (int s = 0; s < 64; ++s) {
+    const int j = FORWARD ? s : 63 - s;
+    double xj[RB];
 #pragma unroll
-    for (int j = 63; j >= 0; --j) {
-      const double xj = bcast(ti * rinv, j);
-      ti = lane == j ? xj : (lane < j ? ti - row[j] * xj : ti);
+    for (int q = 0; q < RB; ++q) xj[q] = bcast(ti[q] * rinv, j);  // lane j scales its entry
+#pragma unroll
+    for (int q = 0; q < RB; ++q) ti[q] = fma(-row[j], xj[q], ti[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < RB; ++q) ti[q] *= rinv;
+}
+
+// Forward push of a row block for RB right-hand sides: part[w][q][i] holds wave
+// w's share of (L21 x_q)(row i); the shares are added in wave order and the sum
+// goes to a later column of the supernode (subtracted from x) or to a row of an
+// ancestor (added to the supernode's contribution vector V).
+template <int RB>
+__device__ inline void push_rows(const double (&part)[kThreads / 64][RB][swc::TM],
+                                 const Panel& sp, const swc::SolveBlock& bk, int rb0, int nq,
+                                 double* __restrict__ X, int64_t ldx, double* __restrict__ V,
+                                 int64_t vld) {
+  for (int e = threadIdx.x; e < RB * swc::TM; e += kThreads) {
+    const int q = e / swc::TM, i = e % swc::TM;
+    if (q < nq && i < bk.nr) {
+      const double y = ((part[0][q][i] + part[1][q][i]) + part[2][q][i]) + part[3][q][i];
+      const int p = (sp.c0 - sp.sc0) + bk.r0 + i;  // position in the supernode's rows
+      if (p < sp.sw) X[(int64_t)(rb0 + q) * ldx + sp.sc0 + p] -= y;
+      else V[(int64_t)(rb0 + q) * vld + sp.voff + (p - sp.sw)] += y;
     }
   }
-  return ti;
 }
 
 // One forward row block of a solve panel (up to SPW = 128 columns of one
-// supernode, two 64-column chunks): solve the panel's diagonal block (every
-// block of the panel, from the same inputs), write the solution if this is the
-// panel's writer block, then push L21 x of the block's rows.  Each wave loads
-// what its step needs at the start, so all loads are in flight at once: wave 0
-// the first chunk's triangle (lane = row), wave 1 the second chunk's rows of the
-// first chunk's columns, wave 2 the second chunk's triangle, and every wave its
-// share of the block's rows.  Then wave 0 substitutes the first chunk, wave 1
-// subtracts its products from the second chunk's entries, wave 2 substitutes
-// the second chunk.
+// supernode, two 64-column chunks) for a group of RB right-hand sides
+// (blockIdx.y; the last group may hold fewer, padded with zero chains): solve
+// the panel's diagonal block (every block of the panel, from the same inputs),
+// write the solution if this is the panel's writer block, then push L21 x of
+// the block's rows.  L is read once per group, not once per right-hand side.
+// Each wave loads what its step needs at the start, so all loads are in flight
+// at once: wave 0 the first chunk's triangle (lane = row), wave 1 the second
+// chunk's rows of the first chunk's columns, wave 2 the second chunk's
+// triangle, and every wave its share of the block's rows.  Then wave 0
+// substitutes the first chunk, wave 1 subtracts its products from the second
+// chunk's entries, wave 2 substitutes the second chunk.
 static_assert(swc::SPW == 128, "the solve kernels split a solve panel in two 64-column chunks");
+template <int RB>
 __global__ void __launch_bounds__(kThreads)
 k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
             const swc::SolveBlock* __restrict__ blocks, int b0, double* __restrict__ V,
             int64_t vld, double* __restrict__ X, double* __restrict__ Y, int nrhs, int64_t ldx,
             int64_t ldy) {
-  __shared__ double xs[swc::SPW];
-  __shared__ double part[kThreads / 64][swc::TM];
+  __shared__ double xs[RB][swc::SPW];
+  __shared__ double part[kThreads / 64][RB][swc::TM];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];
   const Panel sp = spanels[bk.panel];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = sp.ncol, na0 = min(64, nc), na1 = nc - 64;  // chunk widths (na1 <= 0: one chunk)
   const double* D = L + sp.vbase;  // the panel's diagonal block, column j at D + j * ld
-  const int r = blockIdx.y;        // right-hand side
-  double* x = X + (int64_t)r * ldx;
-  double* v = V + (int64_t)r * vld;
+  const int rb0 = blockIdx.y * RB, nq = min(RB, nrhs - rb0);  // this group's right-hand sides
+  // xs[q][c] = x_q of the panel's columns (zero for padded right-hand sides)
+  auto stage = [&]() {
+    for (int e = tid; e < RB * swc::SPW; e += kThreads) {
+      const int q = e / swc::SPW, c = e % swc::SPW;
+      if (c < nc) xs[q][c] = q < nq ? X[(int64_t)(rb0 + q) * ldx + sp.c0 + c] : 0.0;
+    }
+  };
+  auto store = [&]() {
+    for (int e = tid; e < RB * swc::SPW; e += kThreads) {
+      const int q = e / swc::SPW, c = e % swc::SPW;
+      if (q < nq && c < nc) Y[(int64_t)(rb0 + q) * ldy + sp.c0 + c] = xs[q][c];
+    }
+  };
   if (nc <= 64) {
     // one chunk: wave 0 substitutes it, then the block's rows are pushed
-    for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
+    stage();
     __syncthreads();
     if (wave == 0) {
       double row[64];
@@ -612,43 +649,46 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
       }
       const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
       const double rinv = lane < nc ? 1.0 / dval : 1.0;
-      const double ti = substitute64<true>(row, rinv, lane < nc ? xs[lane] : 0.0, lane);
-      if (lane < nc) xs[lane] = ti;
+      double ti[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ti[q] = lane < nc ? xs[q][lane] : 0.0;
+      substitute64<true, RB>(row, rinv, ti);
+      if (lane < nc) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) xs[q][lane] = ti[q];
+      }
     }
     __syncthreads();
-    if (bk.writer)
-      for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
+    if (bk.writer) store();
     if (bk.nr > 0) {
-      double acc = 0.0;
+      double acc[RB] = {};
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int j = wave + 4 * q;
+      for (int k = 0; k < 16; ++k) {
+        const int j = wave + 4 * k;
         const bool in = lane < bk.nr && j < nc;
         const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
-        acc = fma(in ? val : 0.0, j < nc ? xs[j] : 0.0, acc);
+        const double lv = in ? val : 0.0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) acc[q] = fma(lv, j < nc ? xs[q][j] : 0.0, acc[q]);
       }
-      part[wave][lane] = acc;
+#pragma unroll
+      for (int q = 0; q < RB; ++q) part[wave][q][lane] = acc[q];
       __syncthreads();
-      if (wave == 0 && lane < bk.nr) {
-        const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-        const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;
-        if (p < sp.sw) x[sp.sc0 + p] -= y;
-        else v[sp.voff + (p - sp.sw)] += y;
-      }
+      push_rows<RB>(part, sp, bk, rb0, nq, X, ldx, V, vld);
     }
     return;
   }
   // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
   // (entries outside the block read the panel's first diagonal entry, which
   // exists even when the block is empty, and are replaced by zero)
-  constexpr int Q = swc::SPW / 4;
-  double bv[Q];
+  constexpr int NQ = swc::SPW / 4;
+  double bv[NQ];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int j = wave + 4 * q;
+  for (int k = 0; k < NQ; ++k) {
+    const int j = wave + 4 * k;
     const bool in = lane < bk.nr && j < nc;
     const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
-    bv[q] = in ? val : 0.0;
+    bv[k] = in ? val : 0.0;
   }
   // this wave's step: element (row0 + lane, col0 + c), c < 64, where valid
   const int row0 = wave == 0 ? 0 : 64, col0 = wave == 2 ? 64 : 0;
@@ -664,45 +704,56 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const double dval = D[dg ? (int64_t)(row0 + lane) * sp.ld + row0 + lane : 0];
   // reciprocal of the diagonal entry; 1 beyond the chunk (padded unknowns stay 0)
   const double rinv = dg ? 1.0 / dval : 1.0;
-  for (int c = tid; c < nc; c += kThreads) xs[c] = x[sp.c0 + c];
+  stage();
   __syncthreads();
   if (wave == 0) {
-    const double ti = substitute64<true>(m, rinv, lane < na0 ? xs[lane] : 0.0, lane);
-    if (lane < na0) xs[lane] = ti;
+    double ti[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) ti[q] = lane < na0 ? xs[q][lane] : 0.0;
+    substitute64<true, RB>(m, rinv, ti);
+    if (lane < na0) {
+#pragma unroll
+      for (int q = 0; q < RB; ++q) xs[q][lane] = ti[q];
+    }
   }
   __syncthreads();
   if (na1 > 0) {
     if (wave == 1 && lane < na1) {
-      double acc = 0.0;
+      double acc[RB] = {};
 #pragma unroll
-      for (int c = 0; c < 64; ++c) acc = fma(m[c], xs[c], acc);  // na0 = 64 here
-      xs[64 + lane] -= acc;
+      for (int c = 0; c < 64; ++c)
+#pragma unroll
+        for (int q = 0; q < RB; ++q) acc[q] = fma(m[c], xs[q][c], acc[q]);  // na0 = 64 here
+#pragma unroll
+      for (int q = 0; q < RB; ++q) xs[q][64 + lane] -= acc[q];
     }
     __syncthreads();
     if (wave == 2) {
-      const double ti = substitute64<true>(m, rinv, lane < na1 ? xs[64 + lane] : 0.0, lane);
-      if (lane < na1) xs[64 + lane] = ti;
+      double ti[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ti[q] = lane < na1 ? xs[q][64 + lane] : 0.0;
+      substitute64<true, RB>(m, rinv, ti);
+      if (lane < na1) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) xs[q][64 + lane] = ti[q];
+      }
     }
     __syncthreads();
   }
-  if (bk.writer)
-    for (int c = tid; c < nc; c += kThreads) Y[(int64_t)r * ldy + sp.c0 + c] = xs[c];
+  if (bk.writer) store();
   if (bk.nr > 0) {
     // wave w sums columns j = w, w+4, ... for row `lane`, in increasing j
-    double acc = 0.0;
+    double acc[RB] = {};
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int j = wave + 4 * q;
-      acc = fma(bv[q], j < nc ? xs[j] : 0.0, acc);
+    for (int k = 0; k < NQ; ++k) {
+      const int j = wave + 4 * k;
+#pragma unroll
+      for (int q = 0; q < RB; ++q) acc[q] = fma(bv[k], j < nc ? xs[q][j] : 0.0, acc[q]);
     }
-    part[wave][lane] = acc;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) part[wave][q][lane] = acc[q];
     __syncthreads();
-    if (wave == 0 && lane < bk.nr) {
-      const double y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-      const int p = (sp.c0 - sp.sc0) + bk.r0 + lane;  // position in the supernode's rows
-      if (p < sp.sw) x[sp.sc0 + p] -= y;               // a later column of the supernode
-      else v[sp.voff + (p - sp.sw)] += y;              // a row of an ancestor
-    }
+    push_rows<RB>(part, sp, bk, rb0, nq, X, ldx, V, vld);
   }
 }
 
@@ -733,10 +784,9 @@ k_fwd_narrow(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const double rinv = lane < nc ? 1.0 / D[(int64_t)lane * sp.ld + lane] : 1.0;
   double ti = lane < nc ? x[sp.c0 + lane] : 0.0;
 #pragma unroll
-  for (int j = 0; j < NW; ++j) {
-    const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-    ti = lane == j ? xj : (lane > j ? ti - row[j] * xj : ti);
-  }
+  for (int j = 0; j < NW; ++j)  // row[j] = 0 for j >= lane: unmasked steps
+    ti = fma(-row[j], bcast(ti * rinv, j), ti);  // lane j scales its own entry
+  ti *= rinv;
   if (bk.writer && lane < nc) Y[(int64_t)r * ldy + sp.c0 + lane] = ti;
   if (bk.nr > 0) {
     double acc = 0.0;
@@ -805,33 +855,34 @@ k_bwd_diag_narrow(const double* __restrict__ L, const Panel* __restrict__ spanel
     ti = Y[(int64_t)r * ldy + sp.c0 + lane] - sum;
   }
 #pragma unroll
-  for (int j = NW - 1; j >= 0; --j) {
-    const double xj = bcast(ti * rinv, j);  // lane j scales its own entry
-    ti = lane == j ? xj : (lane < j ? ti - col[j] * xj : ti);
-  }
+  for (int j = NW - 1; j >= 0; --j)  // col[j] = 0 for j <= lane: unmasked steps
+    ti = fma(-col[j], bcast(ti * rinv, j), ti);  // lane j scales its own entry
+  ti *= rinv;
   if (lane < nc) X[(int64_t)r * ldx + sp.c0 + lane] = ti;
 }
 
 // Backward row block: partial[j] = sum over the block's rows of L(row, j) x(row)
 // for the solve panel's columns j, into Pb at the block's slot in its level
-// (its index minus the level's first block bl).
+// (its index minus the level's first block bl), for a group of RB right-hand
+// sides (blockIdx.y), so the block of L is read once per group.
 // Thread t takes column t mod SPW and half t / SPW of the block's rows, reads
 // them (contiguous in memory) all at once, and sums them in row order against
 // x(rows) staged in LDS; the two halves are then added, first half first.
 static_assert(kThreads == 2 * swc::SPW && swc::TM % 2 == 0, "two threads per column");
+template <int RB>
 __global__ void __launch_bounds__(kThreads)
 k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
               const Panel* __restrict__ spanels, const swc::SolveBlock* __restrict__ blocks,
               int b0, int bl, double* __restrict__ Pb, int64_t pld, const double* __restrict__ X,
               int nrhs, int64_t ldx) {
   constexpr int H = swc::TM / 2;
-  __shared__ double xr[swc::TM];
-  __shared__ double part[2][swc::SPW];
+  __shared__ double xr[RB][swc::TM];
+  __shared__ double part[2][RB][swc::SPW];
   const swc::SolveBlock bk = blocks[b0 + blockIdx.x];  // nr = 0 writes zeros
   const Panel sp = spanels[bk.panel];
   const int tid = threadIdx.x, c = tid % swc::SPW, h = tid / swc::SPW;
   const int nc = sp.ncol, nr = bk.nr;
-  const int r = blockIdx.y;  // right-hand side
+  const int rb0 = blockIdx.y * RB, nq = min(RB, nrhs - rb0);  // this group's right-hand sides
   // column c, rows [H h, H h + H) of the block (entries outside it read the
   // panel's first diagonal entry, which always exists, and are replaced by zero)
   const double* D = L + sp.vbase;
@@ -843,37 +894,94 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
     const double val = D[in ? off + i : 0];
     lv[i] = in ? val : 0.0;
   }
-  if (tid < nr) xr[tid] = X[(int64_t)r * ldx + Ls[sp.rows_off + bk.r0 + tid]];
+  for (int e = tid; e < RB * swc::TM; e += kThreads) {
+    const int q = e / swc::TM, i = e % swc::TM;
+    if (i < nr)
+      xr[q][i] = q < nq ? X[(int64_t)(rb0 + q) * ldx + Ls[sp.rows_off + bk.r0 + i]] : 0.0;
+  }
   __syncthreads();
-  double acc = 0.0;
+  double acc[RB] = {};
 #pragma unroll
-  for (int i = 0; i < H; ++i) acc = fma(lv[i], H * h + i < nr ? xr[H * h + i] : 0.0, acc);
-  part[h][c] = acc;
+  for (int i = 0; i < H; ++i) {
+    const bool in = H * h + i < nr;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) acc[q] = fma(lv[i], in ? xr[q][H * h + i] : 0.0, acc[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < RB; ++q) part[h][q][c] = acc[q];
   __syncthreads();
-  if (h == 0 && c < nc)
-    Pb[(int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW + c] = part[0][c] + part[1][c];
+  const int64_t slot = (int64_t)(b0 + blockIdx.x - bl) * swc::SPW;
+  for (int e = tid; e < RB * swc::SPW; e += kThreads) {
+    const int q = e / swc::SPW, cc = e % swc::SPW;
+    if (q < nq && cc < nc) Pb[(int64_t)(rb0 + q) * pld + slot + cc] = part[0][q][cc] + part[1][q][cc];
+  }
+}
+
+// t = y - (sum of the panel's row-block partials) for RB right-hand sides, into
+// ts: thread t sums half t / SPW of the blocks for column t mod SPW, in block
+// order, 16 loads in flight per step; the halves are added first half first.
+template <int RB>
+__device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& sp, int f0,
+                                       int f1, int b0, const double* __restrict__ Pb, int64_t pld,
+                                       const double* __restrict__ Y, int64_t ldy, int rb0,
+                                       int nq) {
+  const int tid = threadIdx.x, c = tid % swc::SPW, h = tid / swc::SPW, nc = sp.ncol;
+  const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
+  double sum[RB] = {};
+  if (c < nc) {
+    for (int b = fa; b < fb; b += 16) {
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const double* pb = Pb + (int64_t)(rb0 + min(q, nq - 1)) * pld;
+        double pv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          pv[u] = q < nq && b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) sum[q] += pv[u];
+      }
+    }
+  }
+  if (h == 1) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) ts[q][c] = sum[q];
+  }
+  __syncthreads();
+  if (h == 0 && c < nc) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q)
+      ts[q][c] = q < nq ? Y[(int64_t)(rb0 + q) * ldy + sp.c0 + c] - (sum[q] + ts[q][c]) : 0.0;
+  }
 }
 
 // Backward solve panel: t = y_p - (sum of its row blocks' partials, in block
-// order), then L11^T x_p = t chunk by chunk from the last.  As in the forward
-// kernel each wave loads its step's data at the start: wave 0 the second
+// order), then L11^T x_p = t chunk by chunk from the last, for a group of RB
+// right-hand sides (blockIdx.y).  As in the forward
+// kernel each wave loads its step's operands first: wave 0 the second
 // chunk's transposed triangle (lane = column), wave 1 the second chunk's rows
 // of the first chunk's columns (lane = column), wave 2 the first chunk's
 // transposed triangle.  Wave 0 substitutes the second chunk, wave 1 subtracts
 // its products from the first chunk's entries, wave 2 substitutes the first.
+template <int RB>
 __global__ void __launch_bounds__(kThreads)
 k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
            const int* __restrict__ order, int p0, const int* __restrict__ fblk_ptr, int b0,
            const double* __restrict__ Pb, int64_t pld, const double* __restrict__ Y,
            double* __restrict__ X, int nrhs, int64_t ldx, int64_t ldy) {
-  __shared__ double ts[swc::SPW];
+  __shared__ double ts[RB][swc::SPW];
   const int qi = p0 + blockIdx.x;
   const Panel sp = spanels[order[qi]];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = sp.ncol, na0 = min(64, nc), na1 = nc - 64;  // chunk widths (na1 <= 0: one chunk)
   const int f0 = fblk_ptr[qi], f1 = fblk_ptr[qi + 1];
   const double* D = L + sp.vbase;
-  const int r = blockIdx.y;  // right-hand side
+  const int rb0 = blockIdx.y * RB, nq = min(RB, nrhs - rb0);  // this group's right-hand sides
+  auto store = [&]() {
+    for (int e = tid; e < RB * swc::SPW; e += kThreads) {
+      const int q = e / swc::SPW, c = e % swc::SPW;
+      if (q < nq && c < nc) X[(int64_t)(rb0 + q) * ldx + sp.c0 + c] = ts[q][c];
+    }
+  };
   if (nc <= 64) {
     // one chunk: wave 0 loads its transposed triangle while the partials are
     // summed, then substitutes it
@@ -889,30 +997,20 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
       const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
       rinv = lane < nc ? 1.0 / dval : 1.0;
     }
-    const double* pb = Pb + (int64_t)r * pld;
-    {
-      const int c = tid % swc::SPW, h = tid / swc::SPW;
-      const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
-      double sum = 0.0;
-      if (c < nc) {
-        for (int b = fa; b < fb; b += 16) {
-          double pv[16];
-#pragma unroll
-          for (int u = 0; u < 16; ++u)
-            pv[u] = b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
-#pragma unroll
-          for (int u = 0; u < 16; ++u) sum += pv[u];
-        }
-      }
-      if (h == 1) ts[c] = sum;
-      __syncthreads();
-      if (h == 0 && c < nc) ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - (sum + ts[c]);
-    }
+    gather_partials<RB>(ts, sp, f0, f1, b0, Pb, pld, Y, ldy, rb0, nq);
     __syncthreads();
     if (wave == 0) {
-      const double ti = substitute64<false>(col, rinv, lane < nc ? ts[lane] : 0.0, lane);
-      if (lane < nc) X[(int64_t)r * ldx + sp.c0 + lane] = ti;
+      double ti[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ti[q] = lane < nc ? ts[q][lane] : 0.0;
+      substitute64<false, RB>(col, rinv, ti);
+      if (lane < nc) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) ts[q][lane] = ti[q];
+      }
     }
+    __syncthreads();
+    store();
     return;
   }
   // this wave's step: m[j] = L(rowc + j, colc + lane) where valid
@@ -928,50 +1026,44 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const bool dg = (wave == 0 || wave == 2) && lane < ncl;
   const double dval = D[dg ? (int64_t)(colc + lane) * sp.ld + colc + lane : 0];
   const double rinv = dg ? 1.0 / dval : 1.0;
-  // t = y - (sum of the row blocks' partials): thread t sums half t / SPW of
-  // the blocks for column t mod SPW, in block order, 16 loads in flight per
-  // step; the halves are added first half first
-  const double* pb = Pb + (int64_t)r * pld;
-  {
-    const int c = tid % swc::SPW, h = tid / swc::SPW;
-    const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
-    double sum = 0.0;
-    if (c < nc) {
-      for (int b = fa; b < fb; b += 16) {
-        double pv[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-          pv[u] = b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) sum += pv[u];
-      }
-    }
-    if (h == 1) ts[c] = sum;
-    __syncthreads();
-    if (h == 0 && c < nc) ts[c] = Y[(int64_t)r * ldy + sp.c0 + c] - (sum + ts[c]);
-  }
+  gather_partials<RB>(ts, sp, f0, f1, b0, Pb, pld, Y, ldy, rb0, nq);
   __syncthreads();
   if (na1 > 0) {
     if (wave == 0) {
-      const double ti = substitute64<false>(m, rinv, lane < na1 ? ts[64 + lane] : 0.0, lane);
-      if (lane < na1) ts[64 + lane] = ti;
+      double ti[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ti[q] = lane < na1 ? ts[q][64 + lane] : 0.0;
+      substitute64<false, RB>(m, rinv, ti);
+      if (lane < na1) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) ts[q][64 + lane] = ti[q];
+      }
     }
     __syncthreads();
     // first-chunk entries: t_i -= sum_j L(64 + j, i) x_(64 + j)
     if (wave == 1) {
-      double acc = 0.0;
+      double acc[RB] = {};
 #pragma unroll
-      for (int j = 0; j < 64; ++j) acc = fma(m[j], j < na1 ? ts[64 + j] : 0.0, acc);
-      ts[lane] -= acc;  // na0 = 64 here
+      for (int j = 0; j < 64; ++j)
+#pragma unroll
+        for (int q = 0; q < RB; ++q) acc[q] = fma(m[j], j < na1 ? ts[q][64 + j] : 0.0, acc[q]);
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ts[q][lane] -= acc[q];  // na0 = 64 here
     }
     __syncthreads();
   }
   if (wave == 2) {
-    const double ti = substitute64<false>(m, rinv, lane < na0 ? ts[lane] : 0.0, lane);
-    if (lane < na0) ts[lane] = ti;
+    double ti[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) ti[q] = lane < na0 ? ts[q][lane] : 0.0;
+    substitute64<false, RB>(m, rinv, ti);
+    if (lane < na0) {
+#pragma unroll
+      for (int q = 0; q < RB; ++q) ts[q][lane] = ti[q];
+    }
   }
   __syncthreads();
-  for (int c = tid; c < nc; c += kThreads) X[(int64_t)r * ldx + sp.c0 + c] = ts[c];
+  store();
 }
 
 // y[k] = b[perm[k]] (to the factor's ordering) or x[perm[k]] = y[k] (back).
@@ -1556,8 +1648,11 @@ int factor_device(swc_handle* h, const double* dvals) {
   return SWC_OK;
 }
 
-// Enqueues the forward and backward sweeps on device columns X (factor ordering).
-void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+// Enqueues the forward and backward sweeps on device columns X (factor
+// ordering); the panel kernels take the right-hand sides in groups of RB.
+template <int RB>
+void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+  const int ng_rhs = (nrhs + RB - 1) / RB;
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
@@ -1576,7 +1671,7 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
       hipLaunchKernelGGL(k_fwd_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
                          fbl, b0, h->dV, vld, dX, h->dY, ldx, ldy);
     if (nb > nbn)
-      hipLaunchKernelGGL(k_fwd_panel, dim3(nb - nbn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_fwd_panel<RB>, dim3(nb - nbn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL,
                          sps, fbl, b0 + nbn, h->dV, vld, dX, h->dY, nrhs, ldx, ldy);
   }
   const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
@@ -1587,7 +1682,7 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
       hipLaunchKernelGGL(k_bwd_partial_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0, b0, h->dPb, pld, (const double*)dX, ldx);
     if (nb > nbn)
-      hipLaunchKernelGGL(k_bwd_partial, dim3(nb - nbn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_bwd_partial<RB>, dim3(nb - nbn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0 + nbn, b0, h->dPb, pld, (const double*)dX,
                          nrhs, ldx);
     if (npn > 0)
@@ -1595,10 +1690,17 @@ void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, ldx, ldy);
     if (np > npn)
-      hipLaunchKernelGGL(k_bwd_diag, dim3(np - npn, nrhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+      hipLaunchKernelGGL(k_bwd_diag<RB>, dim3(np - npn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0 + npn, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
+}
+
+// Groups of 4: L is read a quarter as often as with one right-hand side per
+// workgroup, and the kernels still fit their registers (at 8 they spill).
+void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+  if (nrhs >= 2) enqueue_solve_rb<4>(h, dX, nrhs, ldx);
+  else enqueue_solve_rb<1>(h, dX, nrhs, ldx);
 }
 
 // In-place solve of L L^T X = B on device columns X (factor ordering); needs
