@@ -1094,30 +1094,44 @@ __global__ void k_gather(double* __restrict__ out, const double* __restrict__ va
     out[e] = vals[vmap[e]];
 }
 
-// R = B - A X for column blockIdx.y (CSR, one thread per row).
-__global__ void k_residual(const int* __restrict__ rp, const int* __restrict__ ci,
-                           const double* __restrict__ av, const double* __restrict__ X,
-                           const double* __restrict__ B, double* __restrict__ R, int n,
-                           int64_t ld) {
-  const int64_t c = blockIdx.y;
-  const double* x = X + c * ld;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double t = 0.0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) t = fma(av[e], x[ci[e]], t);
-    R[c * ld + i] = B[c * ld + i] - t;
-  }
-}
-
-// y = A x for column blockIdx.y.
-__global__ void k_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
-                       const double* __restrict__ av, const double* __restrict__ X,
-                       double* __restrict__ Y, int n, int64_t ld) {
-  const int64_t c = blockIdx.y;
-  const double* x = X + c * ld;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double t = 0.0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) t = fma(av[e], x[ci[e]], t);
-    Y[c * ld + i] = t;
+// Y = A X, or R = B - A X when B is given, for all nrhs columns (CSR): a group
+// of 8 lanes per row, lane k taking the row's entries k, k+8, ... (consecutive
+// lanes read consecutive entries), right-hand sides 4 at a time against the
+// same entries, each sum completed by a fixed butterfly over the group.  Whole
+// waves step through the rows together (the butterfly needs every lane).
+constexpr int kMvGroup = 8;
+__global__ void __launch_bounds__(kThreads)
+k_csr_mv(const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ av,
+         const double* __restrict__ X, const double* __restrict__ B, double* __restrict__ Y, int n,
+         int64_t ld, int nrhs) {
+  constexpr int G = kMvGroup, RPW = 64 / G;  // rows per wave
+  const int lane = threadIdx.x & 63, k = lane % G;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / 64;
+  for (int64_t base = wid * RPW; base < n; base += nw * RPW) {
+    const int64_t i = base + lane / G;
+    const bool valid = i < n;
+    const int e0 = valid ? rp[i] : 0, e1 = valid ? rp[i + 1] : 0;
+    for (int c0 = 0; c0 < nrhs; c0 += 4) {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int e = e0 + k; e < e1; e += G) {
+        const double a = av[e];
+        const int64_t col = ci[e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (c0 + q < nrhs) acc[q] = fma(a, X[(int64_t)(c0 + q) * ld + col], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int off = G / 2; off >= 1; off /= 2) acc[q] += __shfl_xor(acc[q], off, G);
+      if (valid && k < 4 && c0 + k < nrhs) {
+        // lane k of the group writes right-hand side c0 + k
+        const double t = k == 0 ? acc[0] : k == 1 ? acc[1] : k == 2 ? acc[2] : acc[3];
+        const int64_t o = (int64_t)(c0 + k) * ld + i;
+        Y[o] = B ? B[o] - t : t;
+      }
+    }
   }
 }
 
@@ -1790,6 +1804,17 @@ int reduce_cols(swc_handle* h, const double* X, const double* Y, int ncol, int o
   return SWC_OK;
 }
 
+// Y = A X, or R = B - A X when B is given, on n x nrhs device columns of
+// leading dimension ld (A = the full P A P^T in CSR, uploaded with the pattern).
+void launch_csr_mv(swc_handle* h, const double* X, const double* B, double* Y, int n, int64_t ld,
+                   int nrhs) {
+  const int64_t rows_per_block = kThreads / kMvGroup;
+  const int64_t blocks = std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, 2048);
+  hipLaunchKernelGGL(k_csr_mv, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(kThreads), 0,
+                     h->stream, (const int*)h->dRp, (const int*)h->dCi, (const double*)h->dAf, X, B,
+                     Y, n, ld, nrhs);
+}
+
 int put_coeffs(swc_handle* h, const std::vector<double>& c) {
   SWC_HIP(hipMemcpyAsync(h->dScal + 64 + 2 * c.size(), c.data(), sizeof(double) * c.size(),
                          hipMemcpyHostToDevice, h->stream));
@@ -1817,8 +1842,7 @@ int refine_device(swc_handle* h, const double* Bt, double* Xt, int nrhs, double 
     return hipMemcpyAsync(dst, src, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st);
   };
   auto evaluate = [&](double* Rbuf, bool& done) -> int {
-    hipLaunchKernelGGL(k_residual, g, dim3(kThreads), 0, st, (const int*)h->dRp,
-                       (const int*)h->dCi, (const double*)h->dAf, (const double*)Xt, Bt, Rbuf, n, ld);
+    launch_csr_mv(h, (const double*)Xt, Bt, Rbuf, n, ld, nrhs);
     if (int rc = reduce_cols(h, Rbuf, nullptr, nrhs, 0, rn.data())) return rc;
     if (int rc = reduce_cols(h, Xt, nullptr, nrhs, 0, xn.data())) return rc;
     done = true;
@@ -1854,8 +1878,7 @@ int refine_device(swc_handle* h, const double* Bt, double* Xt, int nrhs, double 
       if (int rc = reduce_cols(h, R, Z, nrhs, 1, rz.data())) return rc;
     }
     for (; !done && it < maxiter; ++it) {
-      hipLaunchKernelGGL(k_spmv, g, dim3(kThreads), 0, st, (const int*)h->dRp, (const int*)h->dCi,
-                         (const double*)h->dAf, (const double*)Pv, Z, n, ld);
+      launch_csr_mv(h, (const double*)Pv, nullptr, Z, n, ld, nrhs);
       if (int rc = reduce_cols(h, Pv, Z, nrhs, 1, pq.data())) return rc;
       for (int c = 0; c < nrhs; ++c) {
         if (!conv[c] && !(pq[c] > 0))
@@ -2232,10 +2255,7 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
   double* R = h->dRef + (int64_t)2 * n * nrhs;
   if (int rc = to_factor(h, B, on_device, ld, Bt, nrhs)) return rc;
   if (int rc = to_factor(h, X, on_device, ld, Xt, nrhs)) return rc;
-  const dim3 g(std::min(grid_for(n), 2048), nrhs);
-  hipLaunchKernelGGL(k_residual, g, dim3(kThreads), 0, h->stream, (const int*)h->dRp,
-                     (const int*)h->dCi, (const double*)h->dAf, (const double*)Xt,
-                     (const double*)Bt, R, n, (int64_t)n);
+  launch_csr_mv(h, (const double*)Xt, (const double*)Bt, R, n, (int64_t)n, nrhs);
   std::vector<double> rn(nrhs), xn(nrhs), bn(nrhs);
   if (int rc = reduce_cols(h, R, nullptr, nrhs, 0, rn.data())) return rc;
   if (int rc = reduce_cols(h, Xt, nullptr, nrhs, 0, xn.data())) return rc;

@@ -61,3 +61,20 @@ def test_fp64_refine_and_gpu_backward_error():
     xp = x + 1e-6 * np.random.default_rng(1).standard_normal(prob.n)
     assert np.isclose(f.backward_error(rhs, xp)[0], O.backward_error(rp, ci, v, xp, rhs),
                       rtol=1e-10)
+
+
+@pytest.mark.parametrize("nrhs", [1, 4, 6])
+def test_backward_error_columns_match_oracle(nrhs):
+    """The residual kernel takes the right-hand sides four at a time against
+    the same row entries; each column's eta matches the oracle's."""
+    prob = P.vector3d_var(6)
+    b, *_ = setup(prob)
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    A, rp, ci, v = full_csr(prob)
+    rng = np.random.default_rng(nrhs)
+    X = rng.uniform(-1, 1, (prob.n, nrhs))
+    B = A @ X + 1e-6 * rng.standard_normal((prob.n, nrhs))
+    eta = f.backward_error(B, X)
+    for j in range(nrhs):
+        assert np.isclose(eta[j], O.backward_error(rp, ci, v, X[:, j], B[:, j]), rtol=1e-10)
