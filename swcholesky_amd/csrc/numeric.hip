@@ -719,13 +719,14 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   __syncthreads();
   if (na1 > 0) {
     if (wave == 1 && lane < na1) {
-      double acc[RB] = {};
 #pragma unroll
-      for (int c = 0; c < 64; ++c)
+      for (int q = 0; q < RB; ++q) {  // one right-hand side at a time, as in the push
+        double acc = 0.0;
 #pragma unroll
-        for (int q = 0; q < RB; ++q) acc[q] = fma(m[c], xs[q][c], acc[q]);  // na0 = 64 here
-#pragma unroll
-      for (int q = 0; q < RB; ++q) xs[q][64 + lane] -= acc[q];
+        for (int c = 0; c < 64; ++c) acc = fma(m[c], xs[q][c], acc);  // na0 = 64 here
+        xs[q][64 + lane] -= acc;
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     __syncthreads();
     if (wave == 2) {
@@ -743,15 +744,20 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   if (bk.writer) store();
   if (bk.nr > 0) {
     // wave w sums columns j = w, w+4, ... for row `lane`, in increasing j
-    double acc[RB] = {};
+    // (one right-hand side at a time: the barrier keeps the scheduler from
+    // hoisting every right-hand side's LDS reads at once, which would not fit
+    // the registers)
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int j = wave + 4 * k;
+    for (int q = 0; q < RB; ++q) {
+      double acc = 0.0;
 #pragma unroll
-      for (int q = 0; q < RB; ++q) acc[q] = fma(bv[k], j < nc ? xs[q][j] : 0.0, acc[q]);
+      for (int k = 0; k < NQ; ++k) {
+        const int j = wave + 4 * k;
+        acc = fma(bv[k], j < nc ? xs[q][j] : 0.0, acc);
+      }
+      part[wave][q][lane] = acc;
+      __builtin_amdgcn_sched_barrier(0);
     }
-#pragma unroll
-    for (int q = 0; q < RB; ++q) part[wave][q][lane] = acc[q];
     __syncthreads();
     push_rows<RB>(part, sp, bk, rb0, nq, X, ldx, V, vld);
   }
@@ -900,15 +906,16 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
       xr[q][i] = q < nq ? X[(int64_t)(rb0 + q) * ldx + Ls[sp.rows_off + bk.r0 + i]] : 0.0;
   }
   __syncthreads();
-  double acc[RB] = {};
+  // one right-hand side at a time (the barrier bounds the LDS reads in flight
+  // to what the registers hold)
 #pragma unroll
-  for (int i = 0; i < H; ++i) {
-    const bool in = H * h + i < nr;
+  for (int q = 0; q < RB; ++q) {
+    double acc = 0.0;
 #pragma unroll
-    for (int q = 0; q < RB; ++q) acc[q] = fma(lv[i], in ? xr[q][H * h + i] : 0.0, acc[q]);
+    for (int i = 0; i < H; ++i) acc = fma(lv[i], H * h + i < nr ? xr[q][H * h + i] : 0.0, acc);
+    part[h][q][c] = acc;
+    __builtin_amdgcn_sched_barrier(0);
   }
-#pragma unroll
-  for (int q = 0; q < RB; ++q) part[h][q][c] = acc[q];
   __syncthreads();
   const int64_t slot = (int64_t)(b0 + blockIdx.x - bl) * swc::SPW;
   for (int e = tid; e < RB * swc::SPW; e += kThreads) {
@@ -920,6 +927,7 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
 // t = y - (sum of the panel's row-block partials) for RB right-hand sides, into
 // ts: thread t sums half t / SPW of the blocks for column t mod SPW, in block
 // order, 16 loads in flight per step; the halves are added first half first.
+// Every column of ts is written (zero past nc and for padded right-hand sides).
 template <int RB>
 __device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& sp, int f0,
                                        int f1, int b0, const double* __restrict__ Pb, int64_t pld,
@@ -932,13 +940,19 @@ __device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& 
     for (int b = fa; b < fb; b += 16) {
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
-        const double* pb = Pb + (int64_t)(rb0 + min(q, nq - 1)) * pld;
-        double pv[16];
+        if (q < nq) {  // uniform
+          const double* pb = Pb + (int64_t)(rb0 + q) * pld + c;
+          double pv[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-          pv[u] = q < nq && b + u < fb ? pb[(int64_t)(b + u - b0) * swc::SPW + c] : 0.0;
+          for (int u = 0; u < 16; ++u) {
+            // clamped to the last block, so every load is inside the partials
+            const double val = pb[(int64_t)(min(b + u, fb - 1) - b0) * swc::SPW];
+            pv[u] = b + u < fb ? val : 0.0;
+          }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) sum[q] += pv[u];
+          for (int u = 0; u < 16; ++u) sum[q] += pv[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight, not 16 RB
       }
     }
   }
@@ -1040,15 +1054,17 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
       }
     }
     __syncthreads();
-    // first-chunk entries: t_i -= sum_j L(64 + j, i) x_(64 + j)
+    // first-chunk entries: t_i -= sum_j L(64 + j, i) x_(64 + j); ts is zero
+    // past nc (gather_partials writes every column), so no read is masked
     if (wave == 1) {
-      double acc[RB] = {};
 #pragma unroll
-      for (int j = 0; j < 64; ++j)
+      for (int q = 0; q < RB; ++q) {  // one right-hand side at a time (register bound)
+        double acc = 0.0;
 #pragma unroll
-        for (int q = 0; q < RB; ++q) acc[q] = fma(m[j], j < na1 ? ts[q][64 + j] : 0.0, acc[q]);
-#pragma unroll
-      for (int q = 0; q < RB; ++q) ts[q][lane] -= acc[q];  // na0 = 64 here
+        for (int j = 0; j < 64; ++j) acc = fma(m[j], ts[q][64 + j], acc);
+        ts[q][lane] -= acc;  // na0 = 64 here
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     __syncthreads();
   }
