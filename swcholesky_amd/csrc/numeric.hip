@@ -1679,10 +1679,11 @@ int factor_device(swc_handle* h, const double* dvals) {
 }
 
 // Enqueues the forward and backward sweeps on device columns X (factor
-// ordering); the panel kernels take the right-hand sides in groups of RB.
-template <int RB>
+// ordering); the forward panel kernel takes the right-hand sides in groups of
+// RF, the backward ones in groups of RB.
+template <int RF, int RB>
 void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  const int ng_rhs = (nrhs + RB - 1) / RB;
+  const int ngf = (nrhs + RF - 1) / RF, ngb = (nrhs + RB - 1) / RB;
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
@@ -1701,7 +1702,7 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
       hipLaunchKernelGGL(k_fwd_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
                          fbl, b0, h->dV, vld, dX, h->dY, ldx, ldy);
     if (nb > nbn)
-      hipLaunchKernelGGL(k_fwd_panel<RB>, dim3(nb - nbn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_fwd_panel<RF>, dim3(nb - nbn, ngf), dim3(kThreads), 0, st, (const double*)h->dL,
                          sps, fbl, b0 + nbn, h->dV, vld, dX, h->dY, nrhs, ldx, ldy);
   }
   const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
@@ -1712,7 +1713,7 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
       hipLaunchKernelGGL(k_bwd_partial_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0, b0, h->dPb, pld, (const double*)dX, ldx);
     if (nb > nbn)
-      hipLaunchKernelGGL(k_bwd_partial<RB>, dim3(nb - nbn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_bwd_partial<RB>, dim3(nb - nbn, ngb), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0 + nbn, b0, h->dPb, pld, (const double*)dX,
                          nrhs, ldx);
     if (npn > 0)
@@ -1720,17 +1721,19 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, ldx, ldy);
     if (np > npn)
-      hipLaunchKernelGGL(k_bwd_diag<RB>, dim3(np - npn, ng_rhs), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+      hipLaunchKernelGGL(k_bwd_diag<RB>, dim3(np - npn, ngb), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0 + npn, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
 }
 
-// Groups of 4: L is read a quarter as often as with one right-hand side per
-// workgroup, and the kernels still fit their registers (at 8 they spill).
+// Groups of right-hand sides: each block of L is read once per group.  The
+// forward kernel stays at 4 (at 8 it needs more than 256 registers per lane,
+// one wave per SIMD); the backward kernels take 8 once there are 16 or more.
 void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  if (nrhs >= 2) enqueue_solve_rb<4>(h, dX, nrhs, ldx);
-  else enqueue_solve_rb<1>(h, dX, nrhs, ldx);
+  if (nrhs >= 16) enqueue_solve_rb<4, 8>(h, dX, nrhs, ldx);
+  else if (nrhs >= 2) enqueue_solve_rb<4, 4>(h, dX, nrhs, ldx);
+  else enqueue_solve_rb<1, 1>(h, dX, nrhs, ldx);
 }
 
 // In-place solve of L L^T X = B on device columns X (factor ordering); needs

@@ -164,7 +164,7 @@ def test_single_supernode_widths(n):
 
 @pytest.mark.parametrize("name,make", [SMALL[3], SMALL[4], SMALL[6]],
                          ids=[SMALL[3][0], SMALL[4][0], SMALL[6][0]])
-@pytest.mark.parametrize("nrhs", [3, 4, 5, 9])
+@pytest.mark.parametrize("nrhs", [3, 4, 5, 9, 17])
 def test_rhs_groups_match_single_rhs(name, make, nrhs):
     """Several right-hand sides are solved in groups (one read of L per group); a
     last group may be partly empty.  Each column of a multi-RHS solve equals the
