@@ -1679,11 +1679,11 @@ int factor_device(swc_handle* h, const double* dvals) {
 }
 
 // Enqueues the forward and backward sweeps on device columns X (factor
-// ordering); the forward panel kernel takes the right-hand sides in groups of
-// RF, the backward ones in groups of RB.
-template <int RF, int RB>
+// ordering); the panel kernels take the right-hand sides in groups: RF for the
+// forward kernel, RP for the backward partials, RD for the backward diagonal.
+template <int RF, int RP, int RD>
 void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  const int ngf = (nrhs + RF - 1) / RF, ngb = (nrhs + RB - 1) / RB;
+  const int ngf = (nrhs + RF - 1) / RF, ngp = (nrhs + RP - 1) / RP, ngd = (nrhs + RD - 1) / RD;
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
@@ -1713,7 +1713,7 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
       hipLaunchKernelGGL(k_bwd_partial_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0, b0, h->dPb, pld, (const double*)dX, ldx);
     if (nb > nbn)
-      hipLaunchKernelGGL(k_bwd_partial<RB>, dim3(nb - nbn, ngb), dim3(kThreads), 0, st, (const double*)h->dL,
+      hipLaunchKernelGGL(k_bwd_partial<RP>, dim3(nb - nbn, ngp), dim3(kThreads), 0, st, (const double*)h->dL,
                          (const int*)h->dLs, sps, fbl, b0 + nbn, b0, h->dPb, pld, (const double*)dX,
                          nrhs, ldx);
     if (npn > 0)
@@ -1721,19 +1721,22 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, ldx, ldy);
     if (np > npn)
-      hipLaunchKernelGGL(k_bwd_diag<RB>, dim3(np - npn, ngb), dim3(kThreads), 0, st, (const double*)h->dL, sps,
+      hipLaunchKernelGGL(k_bwd_diag<RD>, dim3(np - npn, ngd), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0 + npn, (const int*)h->dFblkPtr, b0,
                          (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
   }
 }
 
-// Groups of right-hand sides: each block of L is read once per group.  The
-// forward kernel stays at 4 (at 8 it needs more than 256 registers per lane,
-// one wave per SIMD); the backward kernels take 8 once there are 16 or more.
+// Groups of right-hand sides: each block of L is read once per group, while
+// the substitution chains of a group run one after another in one wave.  The
+// forward kernel takes 4 (at 8 it needs more than 256 registers per lane, one
+// wave per SIMD); the backward partials, which have no chain, take 8, or 16
+// from 16 right-hand sides on; the backward diagonal, mostly chain, takes one
+// per workgroup (measured on the vector 48^3 operator at 8 and 32 RHS).
 void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  if (nrhs >= 16) enqueue_solve_rb<4, 8>(h, dX, nrhs, ldx);
-  else if (nrhs >= 2) enqueue_solve_rb<4, 4>(h, dX, nrhs, ldx);
-  else enqueue_solve_rb<1, 1>(h, dX, nrhs, ldx);
+  if (nrhs >= 16) enqueue_solve_rb<4, 16, 1>(h, dX, nrhs, ldx);
+  else if (nrhs >= 2) enqueue_solve_rb<4, 8, 1>(h, dX, nrhs, ldx);
+  else enqueue_solve_rb<1, 1, 1>(h, dX, nrhs, ldx);
 }
 
 // In-place solve of L L^T X = B on device columns X (factor ordering); needs
