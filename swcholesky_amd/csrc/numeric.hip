@@ -1258,72 +1258,121 @@ __device__ inline int lower_pos(const int* a, int len, int v) {
   return lo;
 }
 
+// The sweep of one level runs in four launches, each spread over blockIdx.y
+// workgroups per supernode (a level near the root holds one or a few large
+// supernodes).  Scratch per supernode: Tt (W x W, Tt[k W + x] = T(k, x), row
+// major), Ut (m x W, Ut[j W + x] = U(j, x), row major) and Zrr (m x m), so that
+// the threads of a wave (consecutive x or i) read consecutive addresses, and
+// the operand shared by the wave is one broadcast load.  Every entry is
+// accumulated in the same order as a one-thread-per-entry loop would.
+
+// T = L11^{-1}, column j per thread: T(j, j) = 1 / L(j, j), then down the rows
+// i > j, T(i, j) = -(sum_{k=j}^{i-1} L(i, k) T(k, j)) / L(i, i).  The wave's
+// columns step through the rows together from its first column j0 (entries
+// T(k, j) with j0 <= k < j are set to zero, so they add nothing), so the loads
+// of L(i, k) are uniform over the wave.
 __global__ void __launch_bounds__(kThreads)
-k_selinv(const double* __restrict__ L, double* __restrict__ Z, const int* __restrict__ Ls,
-         const int* __restrict__ col2sup, const int64_t* __restrict__ spx,
-         const int* __restrict__ ssuper, const int64_t* __restrict__ spi,
-         const SelinvNode* __restrict__ nodes, int node0, double* __restrict__ arena) {
+k_selinv_T(const double* __restrict__ L, const SelinvNode* __restrict__ nodes, int node0,
+           double* __restrict__ arena) {
   const SelinvNode nd = nodes[node0 + blockIdx.x];
-  const int tid = threadIdx.x;
-  const int W = nd.w, R = nd.nrow, m = R - W;
+  const int W = nd.w, R = nd.nrow;
+  const int j = blockIdx.y * kThreads + threadIdx.x;
+  const int j0 = __builtin_amdgcn_readfirstlane(blockIdx.y * kThreads + (threadIdx.x & ~63));
+  if (j0 >= W) return;  // the whole wave
   const double* L11 = L + nd.vbase;  // ld R
-  const double* L21 = L11 + W;
-  double* T = arena + nd.scratch;    // W x W, ld W (column-major, lower)
-  double* U = T + (int64_t)W * W;    // m x W, ld m
-  double* Zrr = U + (int64_t)m * W;  // m x m, ld m (full symmetric)
-  double* Zs = Z + nd.vbase;
+  double* Tt = arena + nd.scratch;
+  const bool mine = j < W;
+  if (mine) {
+    for (int k = j0; k < j; ++k) Tt[(int64_t)k * W + j] = 0.0;
+    Tt[(int64_t)j * W + j] = 1.0 / L11[(int64_t)j * R + j];
+  }
+  for (int i = j0 + 1; i < W; ++i) {
+    double sum = 0.0;
+    for (int k = j0; k < i; ++k) {
+      const double t = mine ? Tt[(int64_t)k * W + j] : 0.0;
+      sum = fma(L11[(int64_t)k * R + i], t, sum);
+    }
+    if (mine && i > j) Tt[(int64_t)i * W + j] = -sum / L11[(int64_t)i * R + i];
+  }
+}
+
+// U = L21 T (entry (i, c): sum over k >= c of L21(i, k) T(k, c)), and Zrr = Z(R, R),
+// both triangles, gathered from the ancestors' selected inverse.
+__global__ void __launch_bounds__(kThreads)
+k_selinv_UZ(const double* __restrict__ L, const double* __restrict__ Z,
+            const int* __restrict__ Ls, const int* __restrict__ col2sup,
+            const int64_t* __restrict__ spx, const int* __restrict__ ssuper,
+            const int64_t* __restrict__ spi, const SelinvNode* __restrict__ nodes, int node0,
+            double* __restrict__ arena) {
+  const SelinvNode nd = nodes[node0 + blockIdx.x];
+  const int W = nd.w, R = nd.nrow, m = R - W;
+  const double* L21 = L + nd.vbase + W;  // ld R
+  const double* Tt = arena + nd.scratch;
+  double* Ut = arena + nd.scratch + (int64_t)W * W;
+  double* Zrr = Ut + (int64_t)m * W;
   const int* rows = Ls + nd.rows_off;
-  // T = L11^{-1}: column j by forward substitution (one thread per column)
-  for (int j = tid; j < W; j += kThreads) {
-    T[(int64_t)j * W + j] = 1.0 / L11[(int64_t)j * R + j];
-    for (int i = 0; i < j; ++i) T[(int64_t)j * W + i] = 0.0;
-    for (int i = j + 1; i < W; ++i) {
+  const int64_t nU = (int64_t)m * W, nZ = (int64_t)m * m;
+  for (int64_t e = (int64_t)blockIdx.y * kThreads + threadIdx.x; e < nU + nZ;
+       e += (int64_t)gridDim.y * kThreads) {
+    if (e < nU) {
+      const int i = (int)(e % m), c = (int)(e / m);
       double sum = 0.0;
-      for (int k = j; k < i; ++k) sum = fma(L11[(int64_t)k * R + i], T[(int64_t)j * W + k], sum);
-      T[(int64_t)j * W + i] = -sum / L11[(int64_t)i * R + i];
+      for (int k = c; k < W; ++k) sum = fma(L21[(int64_t)k * R + i], Tt[(int64_t)k * W + c], sum);
+      Ut[(int64_t)i * W + c] = sum;
+    } else {
+      const int64_t f = e - nU;
+      const int i = (int)(f % m), j = (int)(f / m);
+      if (i < j) continue;
+      const int ri = rows[W + i], rj = rows[W + j];
+      const int K = col2sup[rj];
+      const int rk0 = (int)spi[K];
+      const int lenK = (int)(spi[K + 1] - spi[K]);
+      const int pos = lower_pos(Ls + rk0, lenK, ri);
+      const double z = Z[spx[K] + (int64_t)(rj - ssuper[K]) * lenK + pos];
+      Zrr[(int64_t)j * m + i] = z;
+      Zrr[(int64_t)i * m + j] = z;
     }
   }
-  __syncthreads();
-  // U = L21 T
-  for (int64_t e = tid; e < (int64_t)m * W; e += kThreads) {
+}
+
+// Z21 = -Zrr U, written into Z (rows W.. of the supernode).
+__global__ void __launch_bounds__(kThreads)
+k_selinv_Z21(double* __restrict__ Z, const SelinvNode* __restrict__ nodes, int node0,
+             const double* __restrict__ arena) {
+  const SelinvNode nd = nodes[node0 + blockIdx.x];
+  const int W = nd.w, R = nd.nrow, m = R - W;
+  const double* Ut = arena + nd.scratch + (int64_t)W * W;
+  const double* Zrr = Ut + (int64_t)m * W;
+  double* Zs = Z + nd.vbase;
+  for (int64_t e = (int64_t)blockIdx.y * kThreads + threadIdx.x; e < (int64_t)m * W;
+       e += (int64_t)gridDim.y * kThreads) {
     const int i = (int)(e % m), c = (int)(e / m);
     double sum = 0.0;
-    for (int k = c; k < W; ++k) sum = fma(L21[(int64_t)k * R + i], T[(int64_t)c * W + k], sum);
-    U[(int64_t)c * m + i] = sum;
-  }
-  // Zrr = Z(R, R), both triangles, from the ancestors' selected inverse
-  for (int64_t e = tid; e < (int64_t)m * m; e += kThreads) {
-    const int i = (int)(e % m), j = (int)(e / m);
-    if (i < j) continue;
-    const int ri = rows[W + i], rj = rows[W + j];
-    const int K = col2sup[rj];
-    const int rk0 = (int)spi[K];
-    const int lenK = (int)(spi[K + 1] - spi[K]);
-    const int pos = lower_pos(Ls + rk0, lenK, ri);
-    const double z = Z[spx[K] + (int64_t)(rj - ssuper[K]) * lenK + pos];
-    Zrr[(int64_t)j * m + i] = z;
-    Zrr[(int64_t)i * m + j] = z;
-  }
-  __syncthreads();
-  // Z21 = -Zrr U, written into Z (rows W.. of the supernode)
-  for (int64_t e = tid; e < (int64_t)m * W; e += kThreads) {
-    const int i = (int)(e % m), c = (int)(e / m);
-    double sum = 0.0;
-    for (int j = 0; j < m; ++j) sum = fma(Zrr[(int64_t)j * m + i], U[(int64_t)c * m + j], sum);
+    for (int j = 0; j < m; ++j) sum = fma(Zrr[(int64_t)j * m + i], Ut[(int64_t)j * W + c], sum);
     Zs[(int64_t)c * R + W + i] = -sum;
   }
-  __syncthreads();
-  // Z11 = T^T T - U^T Z21 (lower triangle; strict upper stays zero)
-  for (int64_t e = tid; e < (int64_t)W * W; e += kThreads) {
+}
+
+// Z11 = T^T T - U^T Z21 (lower triangle; the strict upper part stays zero).
+__global__ void __launch_bounds__(kThreads)
+k_selinv_Z11(double* __restrict__ Z, const SelinvNode* __restrict__ nodes, int node0,
+             const double* __restrict__ arena) {
+  const SelinvNode nd = nodes[node0 + blockIdx.x];
+  const int W = nd.w, R = nd.nrow, m = R - W;
+  const double* Tt = arena + nd.scratch;
+  const double* Ut = Tt + (int64_t)W * W;
+  double* Zs = Z + nd.vbase;
+  for (int64_t e = (int64_t)blockIdx.y * kThreads + threadIdx.x; e < (int64_t)W * W;
+       e += (int64_t)gridDim.y * kThreads) {
     const int i = (int)(e % W), c = (int)(e / W);
     if (i < c) {
       Zs[(int64_t)c * R + i] = 0.0;
       continue;
     }
     double sum = 0.0;
-    for (int k = i; k < W; ++k) sum = fma(T[(int64_t)i * W + k], T[(int64_t)c * W + k], sum);
+    for (int k = i; k < W; ++k) sum = fma(Tt[(int64_t)k * W + i], Tt[(int64_t)k * W + c], sum);
     double sub = 0.0;
-    for (int j = 0; j < m; ++j) sub = fma(U[(int64_t)i * m + j], Zs[(int64_t)c * R + W + j], sub);
+    for (int j = 0; j < m; ++j) sub = fma(Ut[(int64_t)j * W + i], Zs[(int64_t)c * R + W + j], sub);
     Zs[(int64_t)c * R + i] = sum - sub;
   }
 }
@@ -1425,6 +1474,8 @@ struct swc_handle {
   int* dQi = nullptr;            // factor-order row / column of every input value
   int* dQj = nullptr;
   std::vector<int> sel_lvl;      // level -> node offsets (root level first)
+  std::vector<int> sel_wmax;     // per level: widest supernode
+  std::vector<int64_t> sel_emax; // per level: most scratch entries of one supernode
   bool zvalid = false;
   int precision = 0;  // 0 FP64; 1 FP32 descendant updates; 2 FP32 for every update
   int fp32_mask = 0;  // bit k: tasks of kind k use FP32 MFMA
@@ -1954,6 +2005,8 @@ int selinv_setup(swc_handle* h) {
   }
   std::vector<SelinvNode> nodes;
   h->sel_lvl.assign(nlev + 1, 0);
+  h->sel_wmax.clear();
+  h->sel_emax.clear();
   int64_t arena = 0;
   for (int l = nlev - 1, k = 0; l >= 0; --l, ++k) {
     int64_t off = 0;
@@ -1972,6 +2025,15 @@ int selinv_setup(swc_handle* h) {
     }
     h->sel_lvl[k + 1] = (int)nodes.size();
     arena = std::max(arena, off);
+    int wmax = 0;
+    int64_t emax = 0;
+    for (int q = h->sel_lvl[k]; q < h->sel_lvl[k + 1]; ++q) {
+      const int64_t w = nodes[q].w, m = nodes[q].nrow - nodes[q].w;
+      wmax = std::max<int>(wmax, (int)w);
+      emax = std::max(emax, std::max(w * w, m * w + m * m));
+    }
+    h->sel_wmax.push_back(wmax);
+    h->sel_emax.push_back(emax);
   }
   int rc = SWC_OK;
   if (!rc) rc = upload(&h->dNodes, nodes.data(), nodes.size(), st);
@@ -1993,11 +2055,22 @@ int selinv_device(swc_handle* h) {
   const int nl = (int)h->sel_lvl.size() - 1;
   for (int k = 0; k < nl; ++k) {
     const int n0 = h->sel_lvl[k], nn = h->sel_lvl[k + 1] - n0;
-    if (nn > 0)
-      hipLaunchKernelGGL(k_selinv, dim3(nn), dim3(kThreads), 0, st, (const double*)h->dL, h->dZ,
-                         (const int*)h->dLs, (const int*)h->dCol2sup, (const int64_t*)h->dSpx,
-                         (const int*)h->dSsuper, (const int64_t*)h->dSpi,
-                         (const SelinvNode*)h->dNodes, n0, h->dArena);
+    if (nn <= 0) continue;
+    const SelinvNode* nd = (const SelinvNode*)h->dNodes;
+    // workgroups per supernode: one per kThreads columns of T; for the other
+    // steps about 8 entries per thread of the level's largest supernode
+    const int gt = (h->sel_wmax[k] + kThreads - 1) / kThreads;
+    const int ge = (int)std::min<int64_t>(std::max<int64_t>((h->sel_emax[k] + 8 * kThreads - 1) / (8 * kThreads), 1), 1024);
+    hipLaunchKernelGGL(k_selinv_T, dim3(nn, gt), dim3(kThreads), 0, st, (const double*)h->dL, nd, n0,
+                       h->dArena);
+    hipLaunchKernelGGL(k_selinv_UZ, dim3(nn, ge), dim3(kThreads), 0, st, (const double*)h->dL,
+                       (const double*)h->dZ, (const int*)h->dLs, (const int*)h->dCol2sup,
+                       (const int64_t*)h->dSpx, (const int*)h->dSsuper, (const int64_t*)h->dSpi, nd,
+                       n0, h->dArena);
+    hipLaunchKernelGGL(k_selinv_Z21, dim3(nn, ge), dim3(kThreads), 0, st, h->dZ, nd, n0,
+                       (const double*)h->dArena);
+    hipLaunchKernelGGL(k_selinv_Z11, dim3(nn, ge), dim3(kThreads), 0, st, h->dZ, nd, n0,
+                       (const double*)h->dArena);
   }
   SWC_HIP(hipGetLastError());
   SWC_HIP(hipStreamSynchronize(st));
