@@ -1266,33 +1266,74 @@ __device__ inline int lower_pos(const int* a, int len, int v) {
 // the operand shared by the wave is one broadcast load.  Every entry is
 // accumulated in the same order as a one-thread-per-entry loop would.
 
-// T = L11^{-1}, column j per thread: T(j, j) = 1 / L(j, j), then down the rows
-// i > j, T(i, j) = -(sum_{k=j}^{i-1} L(i, k) T(k, j)) / L(i, i).  The wave's
-// columns step through the rows together from its first column j0 (entries
-// T(k, j) with j0 <= k < j are set to zero, so they add nothing), so the loads
-// of L(i, k) are uniform over the wave.
-__global__ void __launch_bounds__(kThreads)
+// T = L11^{-1}, column j per lane, one wave per workgroup (64 columns from
+// j0): T(i, j) = 0 above the diagonal, T(j, j) = 1 / L(j, j), and below it
+// T(i, j) = -(sum_{k=j}^{i-1} L(i, k) T(k, j)) / L(i, i), the sum taken in
+// increasing k (the wave's columns start it at j0; the terms k < j are zero).
+// Rows go in blocks of 64: a lane keeps its column's sums for the block's rows
+// in registers (x[r]); the finished rows above the block (k < i0) are applied
+// 64 columns of L at a time from LDS (each load of T(k, j) serves 64 rows),
+// then the block's own triangle, row by row, turns x[r] into T(i0 + r, j).
+__global__ void __launch_bounds__(64)
 k_selinv_T(const double* __restrict__ L, const SelinvNode* __restrict__ nodes, int node0,
            double* __restrict__ arena) {
+  __shared__ double lb[64][64];  // lb[q][r] = L(i0 + r, k0 + q)
+  __shared__ double tl[64][64];  // tl[q][lane] = T(k0 + q, j)
   const SelinvNode nd = nodes[node0 + blockIdx.x];
   const int W = nd.w, R = nd.nrow;
-  const int j = blockIdx.y * kThreads + threadIdx.x;
-  const int j0 = __builtin_amdgcn_readfirstlane(blockIdx.y * kThreads + (threadIdx.x & ~63));
-  if (j0 >= W) return;  // the whole wave
+  const int lane = (int)threadIdx.x;
+  const int j0 = blockIdx.y * 64, j = j0 + lane;
+  if (j0 >= W) return;  // the whole workgroup
   const double* L11 = L + nd.vbase;  // ld R
-  double* Tt = arena + nd.scratch;
+  double* Tt = arena + nd.scratch;   // Tt[k W + j] = T(k, j)
   const bool mine = j < W;
-  if (mine) {
-    for (int k = j0; k < j; ++k) Tt[(int64_t)k * W + j] = 0.0;
-    Tt[(int64_t)j * W + j] = 1.0 / L11[(int64_t)j * R + j];
-  }
-  for (int i = j0 + 1; i < W; ++i) {
-    double sum = 0.0;
-    for (int k = j0; k < i; ++k) {
-      const double t = mine ? Tt[(int64_t)k * W + j] : 0.0;
-      sum = fma(L11[(int64_t)k * R + i], t, sum);
+  for (int i0 = j0; i0 < W; i0 += 64) {
+    const int nb = min(64, W - i0);
+    double x[64];
+#pragma unroll
+    for (int r = 0; r < 64; ++r) x[r] = 0.0;
+    for (int k0 = j0; k0 < i0; k0 += 64) {  // k0 + 63 < i0: whole chunks
+      __syncthreads();  // the previous chunk has been read
+#pragma unroll 16
+      for (int q = 0; q < 64; ++q) {
+        lb[q][lane] = lane < nb ? L11[(int64_t)(k0 + q) * R + i0 + lane] : 0.0;
+        tl[q][lane] = mine ? Tt[(int64_t)(k0 + q) * W + j] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 1
+      for (int q = 0; q < 64; ++q) {
+        const double t = tl[q][lane];
+#pragma unroll
+        for (int r = 0; r < 64; ++r) {
+          x[r] = fma(lb[q][r], t, x[r]);
+          if (r % 16 == 15) __builtin_amdgcn_sched_barrier(0);  // 16 LDS reads in flight
+        }
+      }
     }
-    if (mine && i > j) Tt[(int64_t)i * W + j] = -sum / L11[(int64_t)i * R + i];
+    // the block's triangle: lb[c][r] = L(i0 + r, i0 + c)
+    __syncthreads();
+#pragma unroll 8
+    for (int q = 0; q < 64; ++q)
+      lb[q][lane] = lane < nb && i0 + q < W ? L11[(int64_t)(i0 + q) * R + i0 + lane] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+      const int i = i0 + r;
+      double sum = x[r];
+#pragma unroll
+      for (int c = 0; c < r; ++c) {
+        sum = fma(lb[c][r], x[c], sum);
+        if (c % 16 == 15) __builtin_amdgcn_sched_barrier(0);  // 16 LDS reads in flight
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      const double d = lb[r][r];  // L(i, i) (0 past the block's rows)
+      x[r] = i < j ? 0.0 : (i == j ? 1.0 / d : -sum / d);
+    }
+    if (mine) {
+#pragma unroll
+      for (int r = 0; r < 64; ++r)
+        if (r < nb) Tt[(int64_t)(i0 + r) * W + j] = x[r];
+    }
   }
 }
 
@@ -2057,11 +2098,11 @@ int selinv_device(swc_handle* h) {
     const int n0 = h->sel_lvl[k], nn = h->sel_lvl[k + 1] - n0;
     if (nn <= 0) continue;
     const SelinvNode* nd = (const SelinvNode*)h->dNodes;
-    // workgroups per supernode: one per kThreads columns of T; for the other
+    // workgroups per supernode: one wave per 64 columns of T; for the other
     // steps about 8 entries per thread of the level's largest supernode
-    const int gt = (h->sel_wmax[k] + kThreads - 1) / kThreads;
+    const int gt = (h->sel_wmax[k] + 63) / 64;
     const int ge = (int)std::min<int64_t>(std::max<int64_t>((h->sel_emax[k] + 8 * kThreads - 1) / (8 * kThreads), 1), 1024);
-    hipLaunchKernelGGL(k_selinv_T, dim3(nn, gt), dim3(kThreads), 0, st, (const double*)h->dL, nd, n0,
+    hipLaunchKernelGGL(k_selinv_T, dim3(nn, gt), dim3(64), 0, st, (const double*)h->dL, nd, n0,
                        h->dArena);
     hipLaunchKernelGGL(k_selinv_UZ, dim3(nn, ge), dim3(kThreads), 0, st, (const double*)h->dL,
                        (const double*)h->dZ, (const int*)h->dLs, (const int*)h->dCol2sup,
