@@ -1231,8 +1231,9 @@ __global__ void k_xpby_cols(double* __restrict__ P, const double* __restrict__ Z
 
 // ---- adjoint: selected inverse on the pattern of L ------------------------
 // Z = A^{-1} restricted to the pattern of L (in L's BCSC layout), by the
-// supernodal Takahashi recurrences, supernodes of one level per launch, root
-// level first.  For supernode J with L = [L11; L21] (rows R below):
+// supernodal Takahashi recurrences, one level of supernodes at a time (four
+// launches), root level first.  For supernode J with L = [L11; L21] (rows R
+// below):
 //   T   = L11^{-1}
 //   U   = L21 T                       (recomputed from L, not stored by the factor)
 //   Z21 = -Z(R,R) U                   (Z(R,R) gathered from the ancestors' Z)
@@ -1242,7 +1243,7 @@ __global__ void k_xpby_cols(double* __restrict__ P, const double* __restrict__ Z
 
 struct SelinvNode {
   int64_t vbase;    // value offset of the supernode (row 0, column 0)
-  int64_t scratch;  // offset of its T | U | Zrr scratch
+  int64_t scratch;  // offset of its Tt | Ut | Zrr scratch
   int w, nrow;      // width, rows
   int c0;           // first column
   int rows_off;     // offset into Ls
