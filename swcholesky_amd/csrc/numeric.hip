@@ -807,8 +807,9 @@ k_fwd_narrow(const double* __restrict__ L, const Panel* __restrict__ spanels,
 }
 
 // Backward row block of a narrow panel: partial[j] = sum over the block's rows
-// of L(row, j) x(row), lane = row, each column's products combined by a fixed
-// butterfly.
+// of L(row, j) x(row).  Lane i forms row i's products and stores them in LDS;
+// lane l then sums column l mod 16 over rows 16g..16g+15 (g = l / 16) in order,
+// and two butterfly steps add the four row groups in a fixed order.
 __global__ void __launch_bounds__(64)
 k_bwd_partial_narrow(const double* __restrict__ L, const int* __restrict__ Ls,
                      const Panel* __restrict__ spanels,
@@ -822,16 +823,20 @@ k_bwd_partial_narrow(const double* __restrict__ L, const int* __restrict__ Ls,
   const int nc = sp.ncol;
   const double* blk = L + sp.vbase + bk.r0;
   const double xr = lane < bk.nr ? X[(int64_t)r * ldx + Ls[sp.rows_off + bk.r0 + lane]] : 0.0;
-  double a[NW];
+  static_assert(NW == 16, "lane l sums column l mod 16 of a quarter of the rows");
+  __shared__ double pr[64][NW + 1];  // pr[row][column] (padded against bank conflicts)
 #pragma unroll
-  for (int c = 0; c < NW; ++c) a[c] = (lane < bk.nr && c < nc) ? blk[(int64_t)c * sp.ld + lane] * xr : 0.0;
+  for (int c = 0; c < NW; ++c)
+    pr[lane][c] = (lane < bk.nr && c < nc) ? blk[(int64_t)c * sp.ld + lane] * xr : 0.0;
+  __syncthreads();
+  const int cl = lane % NW, g = lane / NW;
+  double sum = 0.0;
+#pragma unroll
+  for (int rr = 0; rr < 16; ++rr) sum += pr[16 * g + rr][cl];
+  sum += __shfl_xor(sum, 16);
+  sum += __shfl_xor(sum, 32);
   double* out = Pb + (int64_t)r * pld + (int64_t)(b0 + blockIdx.x - bl) * swc::SPW;
-#pragma unroll
-  for (int c = 0; c < NW; ++c) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off);
-    if (lane == 0 && c < nc) out[c] = a[c];
-  }
+  if (lane < nc) out[lane] = sum;  // lanes 0..15 hold columns 0..15
 }
 
 // Backward narrow panel: t = y_p - (sum of its row blocks' partials, in block
