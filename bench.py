@@ -336,7 +336,7 @@ def bench_gmrf(args):
         out.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-    res = {"forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "loss": float(out),
+    res = {"forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "loss": float(out.detach()),
            "n": prob.n, "flops_F": b.flops}
     log(f"gmrf: {res}")
     print(json.dumps({"metric": "gradient of logdet(Q) + b^T Q^-1 b (256^2 GMRF)",

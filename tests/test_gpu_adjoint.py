@@ -18,8 +18,13 @@ def _factor(prob):
     return b, f
 
 
+# dense_spd(200) is one supernode of 200 columns and lap3d 12^3 has separators of up
+# to 144 columns: T = L11^{-1} then spans several 64-row blocks (rows above a block
+# applied from LDS), beside the narrow supernodes of the other cases
 @pytest.mark.parametrize("make", [lambda: P.gmrf2d(32), lambda: P.laplacian3d_var(8),
-                                  lambda: P.vector3d_var(4), lambda: P.laplacian2d_const(30)])
+                                  lambda: P.vector3d_var(4), lambda: P.laplacian2d_const(30),
+                                  lambda: P.dense_spd(200, seed=7),
+                                  lambda: P.laplacian3d_var(12)])
 def test_selected_inverse_matches_dense(make):
     prob = make()
     b, f = _factor(prob)
