@@ -39,6 +39,18 @@ int oracle_factor(int n, const int* c, const int* r, const double* values, const
                   int supNo, const int* prunePtr, const int* pruneSet, int fp32_updates,
                   int nthreads);
 
+/* BLAS-3 form of the same loop: the reference's dsyrk_/dgemm_ per descendant,
+ * dpotrf_/dtrsm_ per target (sw_parallel_PB_Cholesky_05.c:300-466), from the
+ * BLAS/LAPACK library loaded by oracle_blas_load (Fortran ABI, LP64).  Levels of
+ * the descendant DAG with at least nthreads targets run one target per thread
+ * with sequential BLAS; smaller levels run their targets in turn with threaded
+ * BLAS.  Returns 0, k+1 for the first failing column k, -1 on allocation
+ * failure, -2 when no BLAS is loaded. */
+int oracle_blas_load(const char* path);
+int oracle_factor_blas(int n, const int* c, const int* r, const double* values, const int64_t* lC,
+                       const int* Ls, const int64_t* Li_ptr, double* lValues, const int* blockSet,
+                       int supNo, const int* prunePtr, const int* pruneSet, int nthreads);
+
 /* Forward solve L y = b in place (Triangular_BCSC.h:14-49, blockedLsolve), nrhs
  * columns of length n with leading dimension ldx. */
 void oracle_lsolve(int n, const int64_t* lC, const int* Ls, const int64_t* Li_ptr,

@@ -166,7 +166,7 @@ int swc_symbolic_copy(const swc_symbolic* sym, int which, void* dst) {
   if (!sym || !dst) return swc_set_error(SWC_EINVAL, "swc_symbolic_copy: null argument");
   const swc::Symbolic& S = sym->S;
   auto cp = [&](const auto& v) {
-    std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    if (!v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
     return SWC_OK;
   };
   switch (which) {

@@ -1475,6 +1475,7 @@ struct swc_handle {
   // device buffers
   double* dL = nullptr;
   float* dLf = nullptr;  // FP32 shadow of L, written with it when FP32 updates are on
+  int64_t lf_cap = 0;    // elements allocated at dLf
   int* dLs = nullptr;
   Panel* dPanels = nullptr;
   Tile* dTiles = nullptr;
@@ -1504,7 +1505,11 @@ struct swc_handle {
   int64_t* dVmap = nullptr;
   double* dAf = nullptr;
   int64_t nnz_full = 0;
-  double* dScal = nullptr;   // small device scalars (norms, reductions)
+  // small device scalars written by captured launch sequences ([0] ||A||_inf,
+  // [1] logdet): allocated with the structure and never moved while a graph
+  // that writes them exists
+  double* dNorm = nullptr;
+  double* dScal = nullptr;   // refinement scalars, sized by ensure_work (eager launches only)
   double* dNormPart = nullptr;  // kRedBlocks partial maxima of ||A||_inf
   double* dPart = nullptr;   // reduction partials, kRedBlocks * work_cols
   double* dRef = nullptr;    // refinement vectors, 5 * n * work_cols
@@ -1561,14 +1566,15 @@ struct swc_handle {
     (void)hipFree(dL); (void)hipFree(dLf); (void)hipFree(dLs); (void)hipFree(dPanels); (void)hipFree(dTiles); (void)hipFree(dTasks);
     (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dTrsm); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
     (void)hipFree(dVals); (void)hipFree(dWork); (void)hipFree(dInfo);
-    (void)hipFree(dRp); (void)hipFree(dCi); (void)hipFree(dVmap); (void)hipFree(dAf); (void)hipFree(dScal); (void)hipFree(dPart); (void)hipFree(dNormPart);
+    (void)hipFree(dRp); (void)hipFree(dCi); (void)hipFree(dVmap); (void)hipFree(dAf); (void)hipFree(dNorm); (void)hipFree(dScal); (void)hipFree(dPart); (void)hipFree(dNormPart);
     (void)hipFree(dRef);
     (void)hipFree(dZ); (void)hipFree(dArena); (void)hipFree(dNodes); (void)hipFree(dCol2sup); (void)hipFree(dSpx);
     (void)hipFree(dSsuper); (void)hipFree(dSpi); (void)hipFree(dDiagPos); (void)hipFree(dQi); (void)hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
     dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
-    dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
+    dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dNorm = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
+    lf_cap = 0;
     dRef = nullptr;
     dZ = nullptr; dArena = nullptr; dNodes = nullptr; dCol2sup = nullptr; dSpx = nullptr;
     dSsuper = nullptr; dSpi = nullptr; dDiagPos = nullptr; dQi = nullptr; dQj = nullptr;
@@ -1666,7 +1672,7 @@ void enqueue_factor(swc_handle* h, bool profile) {
   hipLaunchKernelGGL(k_norm_inf, dim3(kRedBlocks), dim3(kThreads), 0, st, (const int*)h->dRp,
                      (const double*)h->dAf, h->n, h->dNormPart);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, (const double*)h->dNormPart, 0,
-                     h->dScal);
+                     h->dNorm);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, 0x7fffffff);
   for (int l = 0; l < P.nlevel; ++l) {
     const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0;
@@ -1742,9 +1748,13 @@ int factor_device(swc_handle* h, const double* dvals) {
   SWC_HIP(hipGetLastError());
   const int big = 0x7fffffff;
   int info = 0;
-  SWC_HIP(hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st));
-  SWC_HIP(hipMemcpyAsync(&h->normA, h->dScal, sizeof(double), hipMemcpyDeviceToHost, st));
+  // both copies target host memory of this frame and the handle: the stream is
+  // synchronized before any return, also when a copy fails to enqueue
+  hipError_t ce = hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess)
+    ce = hipMemcpyAsync(&h->normA, h->dNorm, sizeof(double), hipMemcpyDeviceToHost, st);
   SWC_HIP(hipStreamSynchronize(st));
+  SWC_HIP(ce);
   float ms = 0;
   SWC_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
   h->t_factor_ms = ms;
@@ -2170,6 +2180,11 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   h->factored = false;
   h->drop_graphs();
   h->work_cols = 0;  // the solve buffers are sized by the structure: rebuilt on next use
+  // the old structure's L goes first: if this upload fails part way, no buffer
+  // sized for the old structure is left beside the new plan, and swc_factor
+  // refuses to run until an upload completes
+  (void)hipFree(h->dL);
+  h->dL = nullptr;
   h->n = n;
   h->nsuper = supNo;
   h->super.assign(blockSet, blockSet + supNo + 1);
@@ -2243,7 +2258,15 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   hipStream_t st = h->stream;
   int rc = SWC_OK;
   if (!rc) rc = upload(&h->dL, (const double*)nullptr, (size_t)P.xsize, st);
-  if (!rc && h->precision > 0) rc = upload(&h->dLf, (const float*)nullptr, (size_t)P.xsize, st);
+  // the FP32 shadow follows the structure's size whatever the current mode: a
+  // later switch to a mixed mode finds it sized for this structure
+  (void)hipFree(h->dLf);
+  h->dLf = nullptr;
+  h->lf_cap = 0;
+  if (!rc && h->precision > 0) {
+    rc = upload(&h->dLf, (const float*)nullptr, (size_t)P.xsize, st);
+    if (!rc) h->lf_cap = P.xsize;
+  }
   if (!rc) rc = upload(&h->dLs, h->Ls.data(), h->Ls.size(), st);
   if (!rc) rc = upload(&h->dPanels, P.panels.data(), P.panels.size(), st);
   if (!rc) rc = upload(&h->dTiles, P.tiles.data(), P.tiles.size(), st);
@@ -2264,7 +2287,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   if (!rc) rc = upload(&h->dCi, fci.data(), fci.size(), st);
   if (!rc) rc = upload(&h->dVmap, fvm.data(), fvm.size(), st);
   if (!rc) rc = upload(&h->dAf, (const double*)nullptr, fci.size(), st);
-  if (!rc) rc = upload(&h->dScal, (const double*)nullptr, 64, st);
+  if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
   if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);
   if (!rc) rc = upload(&h->dQi, qi.data(), qi.size(), st);
   if (!rc) rc = upload(&h->dQj, qj.data(), qj.size(), st);
@@ -2276,8 +2299,15 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
   h->dZ = nullptr;
   h->dArena = nullptr;
   h->dNodes = nullptr;
-  if (rc) return rc;
-  SWC_HIP(hipStreamSynchronize(st));
+  // the copies above read this frame's vectors: wait for them on every path
+  const hipError_t se = hipStreamSynchronize(st);
+  if (rc) {
+    // a partial upload leaves no usable structure: factor calls are refused
+    (void)hipFree(h->dL);
+    h->dL = nullptr;
+    return rc;
+  }
+  SWC_HIP(se);
   return SWC_OK;
 }
 
@@ -2412,8 +2442,19 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
 int swc_set_precision(swc_handle* h, int mode) {
   if (int rc = check_handle(h)) return rc;
   if (mode < 0 || mode > 2) return swc_set_error(SWC_EINVAL, "swc_set_precision: mode must be 0, 1 or 2");
-  if (mode > 0 && !h->dLf && h->plan.xsize > 0)
+  if (mode > 0 && h->lf_cap < h->plan.xsize) {
+    // a factor graph of a mixed mode captured the old shadow: drop it with the buffer
+    for (int m = 1; m < 3; ++m)
+      if (h->fgraph[m]) {
+        (void)hipGraphExecDestroy(h->fgraph[m]);
+        h->fgraph[m] = nullptr;
+      }
+    (void)hipFree(h->dLf);
+    h->dLf = nullptr;
+    h->lf_cap = 0;
     SWC_HIP(hipMalloc((void**)&h->dLf, sizeof(float) * (size_t)h->plan.xsize));
+    h->lf_cap = h->plan.xsize;
+  }
   h->precision = mode;
   h->fp32_mask = mode == 0 ? 0 : (mode == 1 ? 1 : 3);
   return SWC_OK;
@@ -2439,9 +2480,9 @@ int swc_logdet(swc_handle* h, double* logdet) {
   hipLaunchKernelGGL(k_logdet_partial, dim3(kRedBlocks), dim3(kThreads), 0, st, (const double*)h->dL,
                      (const int64_t*)h->dDiagPos, h->n, h->dNormPart);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, (const double*)h->dNormPart, 1,
-                     h->dScal + 1);
+                     h->dNorm + 1);
   double v = 0;
-  SWC_HIP(hipMemcpyAsync(&v, h->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  SWC_HIP(hipMemcpyAsync(&v, h->dNorm + 1, sizeof(double), hipMemcpyDeviceToHost, st));
   SWC_HIP(hipStreamSynchronize(st));
   *logdet = 2.0 * v;
   return SWC_OK;

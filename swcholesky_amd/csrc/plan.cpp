@@ -516,7 +516,7 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
   if (!p || !dst) return swc_set_error(SWC_EINVAL, "swc_plan_export: null argument");
   const swc::Plan& P = p->P;
   auto cp = [&](const auto& v) {
-    std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    if (!v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
     return SWC_OK;
   };
   switch (which) {

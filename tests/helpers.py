@@ -11,21 +11,6 @@ def setup(prob):
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     a2x = b.a2_values(prob.values)
     L, info = O.factor(b, a2x)
-    if info and prob.name.startswith("gmrf"):  # diagnostic for an SPD operator
-        import os
-        g = np.load(os.path.join(os.path.dirname(__file__), "golden", prob.name + ".npz"))
-        names = {"perm": "Perm", "super": "super", "s": "s", "pi": "pi", "p": "p",
-                 "i_ptr": "i_ptr", "pruneptr": "prunePtr", "pruneset": "pruneSet",
-                 "a2_p": "a2p", "a2_i": "a2i"}
-        diff = [k for k, a in names.items() if not np.array_equal(g[k], getattr(b, a))]
-        print(f"oracle info {info} on {prob.name}; arrays differing from golden: {diff}; "
-              f"a2x equal: {np.array_equal(g['a2_x'], a2x)}; "
-              f"values sum {prob.values.sum():.17g}", flush=True)
-        b2 = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
-        print("fresh analysis equal:", all(np.array_equal(getattr(b, a), getattr(b2, a))
-                                           for a in names.values()), flush=True)
-        Lg, ig = O.factor(b, g["a2_x"], nthreads=1)
-        print("oracle on golden values:", ig, flush=True)
     return b, a2x, L, info
 
 

@@ -36,6 +36,29 @@ def factor(bcsc, a2x, fp32_updates=False, nthreads=0):
     return L, info
 
 
+BLAS_PATH = os.environ.get("SWC_ORACLE_BLAS", "/opt/conda/lib/libmkl_rt.so.1")
+
+
+def blas_available():
+    """True when the BLAS-3 oracle can load its BLAS/LAPACK library."""
+    return os.path.exists(BLAS_PATH) and lib().oracle_blas_load(BLAS_PATH.encode()) == 0
+
+
+def factor_blas(bcsc, a2x, nthreads=0):
+    """The BLAS-3 form of the oracle (the reference's dsyrk/dgemm/dpotrf/dtrsm calls)."""
+    if not blas_available():
+        raise RuntimeError(f"no BLAS for the oracle at {BLAS_PATH}")
+    L = np.zeros(bcsc.xsize, dtype=np.float64)
+    a2x = np.ascontiguousarray(a2x, dtype=np.float64)
+    info = lib().oracle_factor_blas(
+        ctypes.c_int(bcsc.n), _p(bcsc.a2p), _p(bcsc.a2i), _p(a2x), _p(bcsc.p), _p(bcsc.s),
+        _p(bcsc.i_ptr), _p(L), _p(bcsc.super), ctypes.c_int(bcsc.nsuper), _p(bcsc.prunePtr),
+        _p(bcsc.pruneSet), ctypes.c_int(nthreads))
+    if info < 0:
+        raise RuntimeError(f"oracle_factor_blas failed ({info})")
+    return L, info
+
+
 def lsolve(bcsc, L, x):
     x = np.asfortranarray(np.array(x, dtype=np.float64).reshape(bcsc.n, -1))
     lib().oracle_lsolve(ctypes.c_int(bcsc.n), _p(bcsc.p), _p(bcsc.s), _p(bcsc.i_ptr), _p(L),
@@ -69,6 +92,22 @@ def backward_error(rp, ci, v, x, b):
     x = np.ascontiguousarray(x, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     return lib().oracle_backward_error(ctypes.c_int(n), _p(rp), _p(ci), _p(v), _p(x), _p(b))
+
+
+def to_sparse_L(bcsc, L):
+    """Lower factor as a scipy CSC matrix (explicit zeros of the BCSC blocks kept)."""
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    for s in range(bcsc.nsuper):
+        c0, c1 = int(bcsc.super[s]), int(bcsc.super[s + 1])
+        R = bcsc.s[bcsc.pi[s]:bcsc.pi[s + 1]]
+        blk = L[bcsc.p[c0]:bcsc.p[c0] + len(R) * (c1 - c0)].reshape(c1 - c0, len(R))
+        for j in range(c1 - c0):
+            rows.append(R[j:])
+            cols.append(np.full(len(R) - j, c0 + j))
+            vals.append(blk[j, j:])
+    return sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(bcsc.n, bcsc.n))
 
 
 def to_dense_L(bcsc, L):

@@ -180,3 +180,55 @@ def test_rhs_groups_match_single_rhs(name, make, nrhs):
         xj = f.solve(B[:, j])
         assert np.allclose(X[:, j], xj, rtol=1e-13, atol=1e-13 * np.abs(xj).max())
         assert O.backward_error(rp, ci, v, X[:, j], B[:, j]) < 1e-14
+
+
+def test_refactor_after_multi_rhs_solve_uses_new_norm():
+    """A solve with 8 right-hand sides grows the solve workspace after the factor
+    graph was captured; the replayed refactorization must still report ||A||_inf of
+    the new values: backward errors of a perturbed solution agree with the oracle's
+    (which sees only the new values) at rtol 1e-10."""
+    prob = SMALL[3][1]()
+    b, a2x, Lo, _ = setup(prob)
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    B = np.random.default_rng(4).uniform(-1, 1, (prob.n, 8))
+    f.solve(B)
+    v2 = prob.values * 2.0
+    assert f.factor(v2) == 0
+    Lo2, _ = O.factor(b, b.a2_values(v2))
+    assert relF(f.L_values(), Lo2) < 1e-12
+    prob2 = P.Problem(prob.n, prob.colptr, prob.rowidx, v2, prob.perm, "x2")
+    A, rp, ci, v = full_csr(prob2)
+    x, eta, iters = f.refine(B[:, 0], tol=1e-12)
+    assert eta[0] <= 1e-12 and O.backward_error(rp, ci, v, x, B[:, 0]) <= 1e-12
+    xp = x + 1e-6 * np.random.default_rng(5).standard_normal(prob.n)
+    assert np.isclose(f.backward_error(B[:, 0], xp)[0], O.backward_error(rp, ci, v, xp, B[:, 0]),
+                      rtol=1e-10)
+
+
+def test_mixed_after_reupload_of_larger_structure():
+    """mixed -> FP64 -> upload of a larger structure -> mixed: the FP32 copy of L
+    follows the structure, and the mixed factor is the oracle's FP32-update factor
+    up to FP32 rounding."""
+    small, large = SMALL[0][1](), SMALL[3][1]()
+    bs = S.analyze_p2(small.n, small.colptr, small.rowidx, small.perm)
+    bl = S.analyze_p2(large.n, large.colptr, large.rowidx, large.perm)
+    assert bl.xsize > bs.xsize
+    f = S.Factor(bs)
+    f.set_precision(1)
+    assert f.factor(small.values) == 0
+    f.set_precision(0)
+    assert f.factor(small.values) == 0
+    S._lib.check(S._lib.lib().swc_upload_symbolic(f._h, bl._h))
+    f.bcsc, f.n = bl, bl.n
+    f.set_precision(1)
+    assert f.factor(large.values) == 0
+    a2x = bl.a2_values(large.values)
+    L64, _ = O.factor(bl, a2x)
+    L32, _ = O.factor(bl, a2x, fp32_updates=True)
+    Lm = f.L_values()
+    assert relF(Lm, L32) < 10 * max(relF(L32, L64), 1e-7)
+    A, rp, ci, v = full_csr(large)
+    rhs = A @ np.random.default_rng(6).uniform(-1, 1, large.n)
+    x, eta, _ = f.refine(rhs, tol=1e-12)
+    assert eta[0] <= 1e-12 and O.backward_error(rp, ci, v, x, rhs) <= 1e-12

@@ -85,3 +85,122 @@ def test_fp32_updates_are_close():
     L32, info = O.factor(b, a2x, fp32_updates=True)
     assert info == 0
     assert 1e-12 < relF(L32, L64) < 1e-4
+
+
+# ---- backward error: the pass criterion of every GPU solve test ----------
+
+def test_backward_error_hand_cases():
+    """eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) on 2 x 2 cases worked
+    out by hand: A = [[4, 1], [1, 3]], x = (1, -2): A x = (2, -5), ||A||_inf = 5,
+    ||x||_inf = 2.  With b = 0: r = (-2, 5), eta = 5 / 10.  With b = (1, 1):
+    r = (-1, 6), eta = 6 / 11 (a 1-norm, 2-norm or missing ||b|| term gives another
+    value in each case)."""
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    v = np.array([4.0, 1.0, 1.0, 3.0])
+    x = np.array([1.0, -2.0])
+    assert O.backward_error(rp, ci, v, x, np.zeros(2)) == 0.5
+    assert O.backward_error(rp, ci, v, x, np.ones(2)) == 6.0 / 11.0
+    # the exact solution has eta 0
+    assert O.backward_error(rp, ci, v, x, np.array([2.0, -5.0])) == 0.0
+
+
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(10), lambda: P.vector3d_var(6),
+                                  lambda: P.gmrf2d(24)])
+def test_backward_error_matches_numpy(make):
+    """The oracle's eta against an independent scipy/numpy evaluation of the same
+    definition, on perturbed solutions (residuals far above rounding)."""
+    prob = make()
+    A, rp, ci, v = full_csr(prob)
+    rng = np.random.default_rng(5)
+    for scale in (1e-3, 1e-8, 1.0):
+        x = rng.uniform(-1, 1, prob.n)
+        b = A @ x + scale * rng.standard_normal(prob.n)
+        xp = x + scale * rng.standard_normal(prob.n)
+        r = b - A @ xp
+        ref = np.abs(r).max() / (abs(A).sum(axis=1).max() * np.abs(xp).max() + np.abs(b).max())
+        assert np.isclose(O.backward_error(rp, ci, v, xp, b), ref, rtol=1e-13, atol=0)
+
+
+# ---- FP32-update mode: one update against a float32 product --------------
+
+def _arrow(vals):
+    """Lower triangle of a symmetric arrowhead [[d0, 0, b0], [0, d1, b1], [b0, b1, d2]]
+    in the identity ordering.  Without unconditional merges of small supernodes
+    (nrelax = 0, zrelax = 0), merging columns 1 and 2 adds no zero and merging the
+    leaf would, so the partition is {0}, {1, 2}: one update, from the one-column leaf {0} (rows 0, 2)
+    onto entry (2, 2) of the second supernode."""
+    d0, d1, d2, b0, b1 = vals
+    n = 3
+    cp = np.array([0, 2, 4, 5], np.int32)
+    ri = np.array([0, 2, 1, 2, 2], np.int32)
+    va = np.array([d0, b0, d1, b1, d2])
+    return P.Problem(n, cp, ri, va, np.arange(n), "arrow")
+
+
+def test_fp32_update_against_float32_product():
+    """In the FP32-update mode the update of entry (2, 2) is fl32(f f), f = fl32(l20)
+    the float32-rounded leaf entry, subtracted in FP64; in FP64 it is l20^2 in FP64.
+    The target's own POTRF is FP64 in both modes:  L22 = sqrt((d2 - c) - l21^2),
+    l21 = b1 / sqrt(d1)."""
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        d0, d1 = rng.uniform(2, 5, 2)
+        b0, b1 = rng.uniform(-1, 1, 2)
+        d2 = 3.0 + rng.uniform()
+        prob = _arrow((d0, d1, d2, b0, b1))
+        b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm, nrelax=(0, 0, 0),
+                         zrelax=(0.0, 0.0, 0.0))
+        assert list(b.super) == [0, 1, 3] and list(b.Perm) == [0, 1, 2]
+        assert list(b.pruneSet[b.prunePtr[1]:b.prunePtr[2]]) == [0]
+        a2x = b.a2_values(prob.values)
+        L64, i64 = O.factor(b, a2x)
+        L32, i32 = O.factor(b, a2x, fp32_updates=True)
+        assert i64 == 0 and i32 == 0
+        # the leaf has no update: identical in both modes, and l20 = b0 / sqrt(d0) to
+        # rounding (the TRSM multiplies by the reciprocal of the pivot)
+        l20 = L64[b.p[0] + 1]
+        assert L32[b.p[0] + 1] == l20 and np.isclose(l20, b0 / np.sqrt(d0), rtol=1e-15)
+        l21 = b1 / np.sqrt(d1)
+        root = b.p[2] + 1  # column 2 = second column of the target, its diagonal row
+        f = np.float32(l20)
+        c32 = float(np.float32(float(f) * float(f)))  # exact product, rounded once
+        assert L32[root] == np.sqrt((d2 - c32) - l21 * l21)
+        assert L64[root] == np.sqrt((d2 - l20 * l20) - l21 * l21)
+        assert L32[root] != L64[root]
+
+
+# ---- BLAS-3 form of the oracle -------------------------------------------
+
+needs_blas = pytest.mark.skipif(not O.blas_available(), reason="no BLAS library for the oracle")
+
+
+@needs_blas
+@pytest.mark.parametrize("make", [lambda: P.laplacian2d_const(40), lambda: P.laplacian3d_var(14),
+                                  lambda: P.vector3d_var(7), lambda: P.dense_spd(300),
+                                  lambda: P.gmrf2d(32)])
+def test_blas_oracle_matches_scalar_oracle(make):
+    prob = make()
+    b, a2x, L, info = setup(prob)
+    Lb, ib = O.factor_blas(b, a2x, nthreads=4)
+    assert info == 0 and ib == 0
+    assert relF(Lb, L) < 1e-13
+
+
+@needs_blas
+def test_blas_oracle_residual_and_info():
+    """||P A P^T v - L (L^T v)|| / (||A|| ||v||) at rounding level for random v (a
+    dropped update or a wrong index shows up far above it), and the same first
+    failing column as the scalar oracle on an indefinite operator."""
+    prob = P.laplacian3d_var(24)
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    Lb, ib = O.factor_blas(b, b.a2_values(prob.values))
+    assert ib == 0
+    Ap = prob.scipy_full()[b.Perm][:, b.Perm]
+    Ls = O.to_sparse_L(b, Lb)
+    v = np.random.default_rng(2).standard_normal(b.n)
+    err = np.linalg.norm(Ap @ v - Ls @ (Ls.T @ v)) / (abs(Ap).sum(axis=1).max() * np.linalg.norm(v))
+    assert err < 1e-15
+    bad = P.laplacian2d_var(20, 5, shift=-2.0)
+    bb, a2x, _, info = setup(bad)
+    assert info > 0 and O.factor_blas(bb, a2x)[1] == info
