@@ -10,9 +10,9 @@ A step is one FP64 numeric factorization with the values already resident in HBM
 (ColumnCount.h:489-499) over the analysed structure.  The same line also carries
 time-to-1e-12 backward error for the FP64 and mixed paths, the roofline of the
 dominant kernel (HIP events around each launch of it, on its own stream) and the
-CPU oracle timed on a bounded sample of the same operator family.
+reference's CPU path (its BLAS-3 loop, restated) timed on the host cores.
 
-    python bench.py [--steps K] [--warmup W] [--side 64] [--cpu-side 40]
+    python bench.py [--steps K] [--warmup W] [--side 64] [--cpu-side 64]
 """
 from __future__ import annotations
 
@@ -36,8 +36,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(side, nthreads):
-    """The CPU oracle (a restatement of the reference loop) on a side^3 sample."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(side, nthreads, runs=5):
+    """The reference's CPU path, restated: the left-looking supernodal loop with the
+    BLAS-3 calls the reference makes (dsyrk/dgemm per descendant, dpotrf/dtrsm per
+    target, sw_parallel_PB_Cholesky_05.c:300-466; MKL as in its x86 build), OpenMP
+    over the targets of a level with sequential BLAS, threaded BLAS on levels with
+    fewer targets than threads (as parallel_PB_Cholesky_05.h:255-264 runs its last
+    level).  One warm-up, then the median of `runs` factorizations of the full
+    operator."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import swcholesky_amd as S
     from swcholesky_amd import problems as P
@@ -45,13 +62,30 @@ def cpu_baseline(side, nthreads):
     prob = P.laplacian3d_var(side)
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     a2x = b.a2_values(prob.values)
-    t0 = time.perf_counter()
-    _, info = O.factor(b, a2x, nthreads=nthreads)
-    dt = time.perf_counter() - t0
-    assert info == 0
-    return {"value": b.flops / dt / 1e9, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
-            "sample": f"FP64 factor of the same operator family on a {side}^3 grid "
-                      f"(n={prob.n}, F={b.flops:.3e}) in {dt:.1f} s"}
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if not O.blas_available():
+        t0 = time.perf_counter()
+        _, info = O.factor(b, a2x, nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        assert info == 0
+        return {"value": b.flops / dt / 1e9, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+                "sample": f"scalar restatement (no BLAS library found), one FP64 factor of "
+                          f"{side}^3 (n={prob.n}) in {dt:.1f} s", "cpu": _cpu_model(),
+                "nproc": avail}
+    times = []
+    for k in range(runs + 1):
+        t0 = time.perf_counter()
+        _, info = O.factor_blas(b, a2x, nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        assert info == 0, info
+        if k > 0:
+            times.append(dt)
+    med = float(np.median(times))
+    return {"value": b.flops / med / 1e9, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+            "sample": f"BLAS-3 restatement of the reference loop (MKL {O.BLAS_PATH}), FP64 factor "
+                      f"of the full {side}^3 operator (n={prob.n}, F={b.flops:.3e}): 1 warm-up, "
+                      f"median of {runs} = {med:.2f} s (runs {', '.join(f'{t:.2f}' for t in times)})",
+            "cpu": _cpu_model(), "nproc": avail}
 
 
 def main():
@@ -60,7 +94,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=64)
-    ap.add_argument("--cpu-side", type=int, default=40)
+    ap.add_argument("--cpu-side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="factor steps only (for counter runs): no refinement paths, no "

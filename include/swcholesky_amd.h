@@ -87,6 +87,18 @@ int sw_cholesky_left_par_05(int n, int* c, int* r, double* values, size_t* lC, i
                             int* levelPtr, int* levelSet, int nPar, int* parPtr, int* partition,
                             int chunk, int threads, int super_max, int col_max, double* nodCost);
 
+/* ---- reference forward-solve entry point -----------------------------------
+ * Drop-in for blockedLsolve (triangularSolve/Triangular_BCSC.h:14-15; called at
+ * examples/triangularTest02.cpp:195 as blockedLsolve(n, L->p, L->s, valL, nnz,
+ * L->i_ptr, L->col2Sup, L->super, L->nsuper, x)): solves L y = x in place on the
+ * host vector x on the current HIP device.  sup2col has supNo+1 entries; NNZ and
+ * col2sup are not read (nor does the reference read them).  The solve schedule is
+ * derived from the row lists and kept for later calls with an equal structure.
+ * Returns 1 on success and 0 on failure, as the reference does (0 when Lp, Li or
+ * x is null; swc_last_error() says why otherwise). */
+int blockedLsolve(int n, size_t* Lp, int* Li, double* Lx, int NNZ, size_t* Li_ptr, int* col2sup,
+                  int* sup2col, int supNo, double* x);
+
 /* ---- handle API (structure uploaded once, refactor / solve many times) ------ */
 typedef struct swc_handle swc_handle;
 
@@ -182,7 +194,9 @@ int swc_get_timings(const swc_handle* h, double* out);
  * offsets, 12 gathered columns by solve level, 13 solve level->gathered-column
  * offsets, 14 solve panels (Panel structs), 15 solve panels by level, 16 solve
  * level->solve-panel offsets) as the packed structs and arrays of
- * swcholesky_amd/csrc/plan.h. */
+ * swcholesky_amd/csrc/plan.h.  With prunePtr NULL the descendant sets are derived
+ * from the row lists (supernode s depends on every d whose rows below its own
+ * columns meet s's columns), as blockedLsolve derives them. */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,

@@ -4,7 +4,8 @@ Host analysis (a restatement of the reference's analyze_p2 for a given ordering)
 hand-written HIP kernels for gfx950 for the numeric factorization and the
 triangular solves, all behind the C ABI of include/swcholesky_amd.h.
 """
-from .api import BCSC, Factor, analyze_p2, sw_cholesky_left_par_05  # noqa: F401
+from .api import BCSC, Factor, analyze_p2, blockedLsolve, sw_cholesky_left_par_05  # noqa: F401
 from ._lib import SwcError, lib  # noqa: F401
 
-__all__ = ["BCSC", "Factor", "analyze_p2", "sw_cholesky_left_par_05", "SwcError", "lib"]
+__all__ = ["BCSC", "Factor", "analyze_p2", "blockedLsolve", "sw_cholesky_left_par_05", "SwcError",
+           "lib"]

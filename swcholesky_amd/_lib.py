@@ -29,6 +29,7 @@ SIGNATURES = {
     "sw_cholesky_left_par_05": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                         c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp,
                                         c_vp, c_int, c_int, c_int, c_int, c_vp]),
+    "blockedLsolve": (c_int, [c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp]),
     "swc_create": (c_int, [ctypes.POINTER(c_vp)]),
     "swc_destroy": (c_int, [c_vp]),
     "swc_upload": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -114,6 +114,20 @@ def sw_cholesky_left_par_05(n, c, r, values, lC, Ls, Li_ptr, lValues, blockSet, 
         int(threads), int(super_max), int(col_max), ptr(nodCost))
 
 
+def blockedLsolve(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x):
+    """The reference forward solve L y = x (Triangular_BCSC.h:14-49) on the GPU, in
+    place on the host array x; returns 1 on success and 0 on failure."""
+    Lp = np.ascontiguousarray(Lp, dtype=np.uint64)
+    Li = np.ascontiguousarray(Li, dtype=np.int32)
+    Lx = np.ascontiguousarray(Lx, dtype=np.float64)
+    Li_ptr = np.ascontiguousarray(Li_ptr, dtype=np.uint64)
+    sup2col = np.ascontiguousarray(sup2col, dtype=np.int32)
+    col2sup = None if col2sup is None else np.ascontiguousarray(col2sup, dtype=np.int32)
+    assert x.dtype == np.float64 and x.flags.c_contiguous and x.size >= n
+    return _lib.lib().blockedLsolve(int(n), ptr(Lp), ptr(Li), ptr(Lx), int(NNZ), ptr(Li_ptr),
+                                    ptr(col2sup), ptr(sup2col), int(supNo), ptr(x))
+
+
 def _dptr(x):
     """(pointer, on_device) of a numpy array or a torch tensor."""
     if isinstance(x, np.ndarray):
@@ -286,8 +300,10 @@ SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("writer", np.int32)])
 
 
-def build_plan(bcsc: BCSC):
-    """The device work plan for a structure, as numpy record arrays (host only)."""
+def build_plan(bcsc: BCSC, derive_sets=False):
+    """The device work plan for a structure, as numpy record arrays (host only).
+    derive_sets: ignore the analysis's descendant sets and derive them from the
+    row lists (what blockedLsolve does)."""
     lib = _lib.lib()
     sizes = np.zeros(4, dtype=np.int64)
     check(lib.swc_plan_struct_sizes(ptr(sizes)))
@@ -298,8 +314,9 @@ def build_plan(bcsc: BCSC):
     super_ = np.ascontiguousarray(bcsc.super)
     p = np.ascontiguousarray(bcsc.p.astype(np.uint64))
     iptr = np.ascontiguousarray(bcsc.i_ptr.astype(np.uint64))
+    pp, ps = (None, None) if derive_sets else (bcsc.prunePtr, bcsc.pruneSet)
     check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
-                             ptr(bcsc.prunePtr), ptr(bcsc.pruneSet), ctypes.byref(h)))
+                             ptr(pp), ptr(ps), ctypes.byref(h)))
     try:
         sz = np.zeros(12, dtype=np.int64)
         fl = (ctypes.c_double * 2)()

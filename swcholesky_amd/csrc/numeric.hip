@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1533,6 +1534,7 @@ struct swc_handle {
   int fp32_mask = 0;  // bit k: tasks of kind k use FP32 MFMA
   int factored_precision = 0;
   bool factored = false;
+  bool has_values = false;  // the A2 pattern (value map, residual CSR) is uploaded
   int last_info = 0;
   // captured launch sequences (hipGraph): factor per precision mode, solves per
   // (buffer, nrhs, ld); rebuilt when the structure changes
@@ -1789,50 +1791,70 @@ int factor_device(swc_handle* h, const double* dvals) {
 // Enqueues the forward and backward sweeps on device columns X (factor
 // ordering); the panel kernels take the right-hand sides in groups: RF for the
 // forward kernel, RP for the backward partials, RD for the backward diagonal.
-template <int RF, int RP, int RD>
-void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  const int ngf = (nrhs + RF - 1) / RF, ngp = (nrhs + RP - 1) / RP, ngd = (nrhs + RD - 1) / RD;
+// Forward sweep L Y = X: X is consumed (it accumulates the pushes), the
+// solution goes to Y (ld n), V holds the contribution vectors (ld vsize).
+template <int RF>
+void enqueue_forward(swc_handle* h, double* dX, int nrhs, int64_t ldx, double* dY, double* dV) {
+  const int ngf = (nrhs + RF - 1) / RF;
   hipStream_t st = h->stream;
   const Plan& P = h->plan;
   const int64_t vld = std::max<int64_t>(P.vsize, 1);
   const int64_t ldy = h->n;
   const Panel* sps = (const Panel*)h->dSpanels;
   const swc::SolveBlock* fbl = (const swc::SolveBlock*)h->dFblk;
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, h->dV, vld * nrhs, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(vld * nrhs)), dim3(kThreads), 0, st, dV, vld * nrhs, 0.0);
   for (int l = 0; l < P.nslevel; ++l) {
     const int g0 = P.lvl_gcol[l], ng = P.lvl_gcol[l + 1] - g0;
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0, nbn = P.lvl_fblk_narrow[l];
     if (ng > 0)
       hipLaunchKernelGGL(k_fwd_gather, dim3((ng + kThreads - 1) / kThreads, nrhs), dim3(kThreads), 0, st,
                          (const int*)h->dGcols, g0, ng, (const int64_t*)h->dGptr,
-                         (const int64_t*)h->dGidx, (const double*)h->dV, vld, dX, nrhs, ldx);
+                         (const int64_t*)h->dGidx, (const double*)dV, vld, dX, nrhs, ldx);
     if (nbn > 0)
       hipLaunchKernelGGL(k_fwd_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
-                         fbl, b0, h->dV, vld, dX, h->dY, ldx, ldy);
+                         fbl, b0, dV, vld, dX, dY, ldx, ldy);
     if (nb > nbn)
       hipLaunchKernelGGL(k_fwd_panel<RF>, dim3(nb - nbn, ngf), dim3(kThreads), 0, st, (const double*)h->dL,
-                         sps, fbl, b0 + nbn, h->dV, vld, dX, h->dY, nrhs, ldx, ldy);
+                         sps, fbl, b0 + nbn, dV, vld, dX, dY, nrhs, ldx, ldy);
   }
+}
+
+// Backward sweep L^T X = Y (Y ld n), into X; Pb holds the row blocks' partials.
+template <int RP, int RD>
+void enqueue_backward(swc_handle* h, double* dX, int nrhs, int64_t ldx, const double* dY,
+                      double* dPb) {
+  const int ngp = (nrhs + RP - 1) / RP, ngd = (nrhs + RD - 1) / RD;
+  hipStream_t st = h->stream;
+  const Plan& P = h->plan;
+  const int64_t ldy = h->n;
+  const Panel* sps = (const Panel*)h->dSpanels;
+  const swc::SolveBlock* fbl = (const swc::SolveBlock*)h->dFblk;
   const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
   for (int l = P.nslevel - 1; l >= 0; --l) {
     const int p0 = P.lvl_sp[l], np = P.lvl_sp[l + 1] - p0, npn = P.lvl_sp_narrow[l];
     const int b0 = P.lvl_fblk[l], nb = P.lvl_fblk[l + 1] - b0, nbn = P.lvl_fblk_narrow[l];
     if (nbn > 0)
       hipLaunchKernelGGL(k_bwd_partial_narrow, dim3(nbn, nrhs), dim3(64), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, sps, fbl, b0, b0, h->dPb, pld, (const double*)dX, ldx);
+                         (const int*)h->dLs, sps, fbl, b0, b0, dPb, pld, (const double*)dX, ldx);
     if (nb > nbn)
       hipLaunchKernelGGL(k_bwd_partial<RP>, dim3(nb - nbn, ngp), dim3(kThreads), 0, st, (const double*)h->dL,
-                         (const int*)h->dLs, sps, fbl, b0 + nbn, b0, h->dPb, pld, (const double*)dX,
+                         (const int*)h->dLs, sps, fbl, b0 + nbn, b0, dPb, pld, (const double*)dX,
                          nrhs, ldx);
     if (npn > 0)
       hipLaunchKernelGGL(k_bwd_diag_narrow, dim3(npn, nrhs), dim3(64), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0, (const int*)h->dFblkPtr, b0,
-                         (const double*)h->dPb, pld, (const double*)h->dY, dX, ldx, ldy);
+                         (const double*)dPb, pld, dY, dX, ldx, ldy);
     if (np > npn)
       hipLaunchKernelGGL(k_bwd_diag<RD>, dim3(np - npn, ngd), dim3(kThreads), 0, st, (const double*)h->dL, sps,
                          (const int*)h->dSpOrder, p0 + npn, (const int*)h->dFblkPtr, b0,
-                         (const double*)h->dPb, pld, (const double*)h->dY, dX, nrhs, ldx, ldy);
+                         (const double*)dPb, pld, dY, dX, nrhs, ldx, ldy);
   }
+}
+
+template <int RF, int RP, int RD>
+void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
+  enqueue_forward<RF>(h, dX, nrhs, ldx, h->dY, h->dV);
+  enqueue_backward<RP, RD>(h, dX, nrhs, ldx, h->dY, h->dPb);
 }
 
 // Groups of right-hand sides: each block of L is read once per group, while
@@ -2140,6 +2162,118 @@ int check_handle(const swc_handle* h) {
   return SWC_OK;
 }
 
+// The BCSC structure the kernels index: every supernode's row list holds its
+// own columns first, then strictly increasing rows below them, all in [0, n);
+// its value block (ld = row count) lies at a non-negative offset.
+int validate_structure(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
+                       const size_t* Li_ptr) {
+  if (n <= 0 || supNo <= 0 || !blockSet || !lC || !Ls || !Li_ptr)
+    return swc_set_error(SWC_EINVAL, "structure: bad argument");
+  if (blockSet[0] != 0 || blockSet[supNo] != n)
+    return swc_set_error(SWC_EINVAL, "structure: blockSet must span 0..n");
+  for (int s = 0; s < supNo; ++s) {
+    const int c0 = blockSet[s], c1 = blockSet[s + 1];
+    if (c1 <= c0) return swc_set_error(SWC_EINVAL, "structure: empty supernode");
+    const size_t p0 = Li_ptr[c0], p1 = Li_ptr[c1];
+    if (p1 < p0 + (size_t)(c1 - c0) || p1 - p0 > 0x7fffffffu)
+      return swc_set_error(SWC_EINVAL, "structure: row list shorter than the supernode");
+    if (lC[c0] > ((size_t)1 << 62)) return swc_set_error(SWC_EINVAL, "structure: bad value offset");
+    for (int c = c0; c < c1; ++c)
+      if (Ls[p0 + (c - c0)] != c)
+        return swc_set_error(SWC_EINVAL, "structure: a supernode's rows must start with its columns");
+    for (size_t i = p0 + (c1 - c0); i < p1; ++i)
+      if (Ls[i] <= Ls[i - 1] || Ls[i] >= n)
+        return swc_set_error(SWC_EINVAL, "structure: rows must increase and stay below n");
+  }
+  return SWC_OK;
+}
+
+// Structure part of an upload: the plan (solve part only without `factor`), L
+// and its FP32 copy, the row lists and the plan arrays on the device.
+int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const size_t* lC,
+                     const int* Ls, const size_t* Li_ptr, const int* prunePtr,
+                     const int* pruneSet, bool factor) {
+  if (int rc = validate_structure(n, blockSet, supNo, lC, Ls, Li_ptr)) return rc;
+  // pruneSet may be null when it is empty (no supernode has descendants)
+  if (!prunePtr || (!pruneSet && prunePtr[supNo] != 0) || prunePtr[0] != 0)
+    return swc_set_error(SWC_EINVAL, "structure: bad descendant sets");
+  for (int s = 0; s < supNo; ++s)
+    if (prunePtr[s + 1] < prunePtr[s]) return swc_set_error(SWC_EINVAL, "structure: bad descendant sets");
+  h->factored = false;
+  h->has_values = false;
+  h->drop_graphs();
+  h->work_cols = 0;  // the solve buffers are sized by the structure: rebuilt on next use
+  // the old structure's L goes first: if this upload fails part way, no buffer
+  // sized for the old structure is left beside the new plan, and swc_factor
+  // refuses to run until an upload completes
+  (void)hipFree(h->dL);
+  h->dL = nullptr;
+  h->n = n;
+  h->nsuper = supNo;
+  h->nvals = 0;
+  h->super.assign(blockSet, blockSet + supNo + 1);
+  h->pi.resize(supNo + 1);
+  h->px.resize(supNo);
+  for (int s = 0; s <= supNo; ++s) h->pi[s] = (int64_t)Li_ptr[blockSet[s]];
+  for (int s = 0; s < supNo; ++s) h->px[s] = (int64_t)lC[blockSet[s]];
+  const int64_t ssize = h->pi[supNo];
+  h->Ls.assign(Ls, Ls + ssize);
+  h->pruneptr.assign(prunePtr, prunePtr + supNo + 1);
+  h->pruneset.assign(pruneSet, pruneSet + prunePtr[supNo]);
+  try {
+    swc::build_plan(n, supNo, h->super.data(), h->pi.data(), h->px.data(), h->Ls.data(),
+                    prunePtr, pruneSet, h->plan, factor);
+  } catch (const std::bad_alloc&) {
+    return swc_set_error(SWC_ENOMEM, "upload: out of host memory");
+  } catch (const std::exception& e) {
+    return swc_set_error(SWC_EINVAL, std::string("upload: ") + e.what());
+  }
+  const Plan& P = h->plan;
+  hipStream_t st = h->stream;
+  int rc = SWC_OK;
+  if (!rc) rc = upload(&h->dL, (const double*)nullptr, (size_t)P.xsize, st);
+  // the FP32 shadow follows the structure's size whatever the current mode: a
+  // later switch to a mixed mode finds it sized for this structure
+  (void)hipFree(h->dLf);
+  h->dLf = nullptr;
+  h->lf_cap = 0;
+  if (!rc && h->precision > 0 && factor) {
+    rc = upload(&h->dLf, (const float*)nullptr, (size_t)P.xsize, st);
+    if (!rc) h->lf_cap = P.xsize;
+  }
+  if (!rc) rc = upload(&h->dLs, h->Ls.data(), h->Ls.size(), st);
+  if (!rc) rc = upload(&h->dPanels, P.panels.data(), P.panels.size(), st);
+  if (!rc) rc = upload(&h->dTiles, P.tiles.data(), P.tiles.size(), st);
+  if (!rc) rc = upload(&h->dTasks, P.tasks.data(), P.tasks.size(), st);
+  if (!rc) rc = upload(&h->dGptr, P.gptr.data(), P.gptr.size(), st);
+  if (!rc) rc = upload(&h->dGidx, P.gidx.data(), P.gidx.size(), st);
+  if (!rc) rc = upload(&h->dGcols, P.gcols.data(), P.gcols.size(), st);
+  if (!rc) rc = upload(&h->dFblk, P.fblocks.data(), P.fblocks.size(), st);
+  if (!rc) rc = upload(&h->dFblkPtr, P.fblk_ptr.data(), P.fblk_ptr.size(), st);
+  if (!rc) rc = upload(&h->dSpanels, P.spanels.data(), P.spanels.size(), st);
+  if (!rc) rc = upload(&h->dSpOrder, P.sp_order.data(), P.sp_order.size(), st);
+  if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
+  if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
+  if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
+  if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
+  if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);
+  h->zvalid = false;
+  (void)hipFree(h->dZ);
+  (void)hipFree(h->dArena);
+  (void)hipFree(h->dNodes);
+  h->dZ = nullptr;
+  h->dArena = nullptr;
+  h->dNodes = nullptr;
+  const hipError_t se = hipStreamSynchronize(st);
+  if (rc) {
+    (void)hipFree(h->dL);
+    h->dL = nullptr;
+    return rc;
+  }
+  SWC_HIP(se);
+  return SWC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2171,42 +2305,12 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
                const int* Ls, const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                const int* A2p, const int* A2i, const int64_t* a2src, int64_t nvals) {
   if (int rc = check_handle(h)) return rc;
-  // pruneSet may be null when it is empty (no supernode has descendants)
-  if (n <= 0 || supNo <= 0 || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr ||
-      (!pruneSet && prunePtr[supNo] != 0) || !A2p || !A2i)
-    return swc_set_error(SWC_EINVAL, "swc_upload: bad argument");
-  if (blockSet[0] != 0 || blockSet[supNo] != n)
-    return swc_set_error(SWC_EINVAL, "swc_upload: blockSet must span 0..n");
-  h->factored = false;
-  h->drop_graphs();
-  h->work_cols = 0;  // the solve buffers are sized by the structure: rebuilt on next use
-  // the old structure's L goes first: if this upload fails part way, no buffer
-  // sized for the old structure is left beside the new plan, and swc_factor
-  // refuses to run until an upload completes
-  (void)hipFree(h->dL);
-  h->dL = nullptr;
-  h->n = n;
-  h->nsuper = supNo;
-  h->super.assign(blockSet, blockSet + supNo + 1);
-  h->pi.resize(supNo + 1);
-  h->px.resize(supNo);
-  for (int s = 0; s <= supNo; ++s) h->pi[s] = (int64_t)Li_ptr[blockSet[s]];
-  for (int s = 0; s < supNo; ++s) {
-    if (blockSet[s + 1] <= blockSet[s]) return swc_set_error(SWC_EINVAL, "swc_upload: empty supernode");
-    h->px[s] = (int64_t)lC[blockSet[s]];
-  }
-  const int64_t ssize = h->pi[supNo];
-  h->Ls.assign(Ls, Ls + ssize);
-  h->pruneptr.assign(prunePtr, prunePtr + supNo + 1);
-  h->pruneset.assign(pruneSet, pruneSet + prunePtr[supNo]);
-  try {
-    swc::build_plan(n, supNo, h->super.data(), h->pi.data(), h->px.data(), h->Ls.data(),
-                    prunePtr, pruneSet, h->plan);
-  } catch (const std::bad_alloc&) {
-    return swc_set_error(SWC_ENOMEM, "swc_upload: out of host memory");
-  } catch (const std::exception& e) {
-    return swc_set_error(SWC_EINVAL, std::string("swc_upload: ") + e.what());
-  }
+  if (!A2p || !A2i) return swc_set_error(SWC_EINVAL, "swc_upload: bad argument");
+  if (int rc = upload_structure(h, n, blockSet, supNo, lC, Ls, Li_ptr, prunePtr, pruneSet, true))
+    return rc;
+  if (A2p[0] != 0 || A2p[n] < 0) return swc_set_error(SWC_EINVAL, "swc_upload: bad A2 pointers");
+  for (int j = 0; j < n; ++j)
+    if (A2p[j + 1] < A2p[j]) return swc_set_error(SWC_EINVAL, "swc_upload: bad A2 pointers");
   h->nvals = a2src ? nvals : (int64_t)A2p[n];
   std::vector<int64_t> amap;
   if (int rc = build_amap(h, A2p, A2i, a2src, nvals, amap)) return rc;
@@ -2254,51 +2358,17 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
     for (int c = blockSet[s]; c < blockSet[s + 1]; ++c)
       dpos[c] = h->px[s] + (int64_t)(c - blockSet[s]) * R + (c - blockSet[s]);
   }
-  const Plan& P = h->plan;
   hipStream_t st = h->stream;
   int rc = SWC_OK;
-  if (!rc) rc = upload(&h->dL, (const double*)nullptr, (size_t)P.xsize, st);
-  // the FP32 shadow follows the structure's size whatever the current mode: a
-  // later switch to a mixed mode finds it sized for this structure
-  (void)hipFree(h->dLf);
-  h->dLf = nullptr;
-  h->lf_cap = 0;
-  if (!rc && h->precision > 0) {
-    rc = upload(&h->dLf, (const float*)nullptr, (size_t)P.xsize, st);
-    if (!rc) h->lf_cap = P.xsize;
-  }
-  if (!rc) rc = upload(&h->dLs, h->Ls.data(), h->Ls.size(), st);
-  if (!rc) rc = upload(&h->dPanels, P.panels.data(), P.panels.size(), st);
-  if (!rc) rc = upload(&h->dTiles, P.tiles.data(), P.tiles.size(), st);
-  if (!rc) rc = upload(&h->dTasks, P.tasks.data(), P.tasks.size(), st);
-  if (!rc) rc = upload(&h->dGptr, P.gptr.data(), P.gptr.size(), st);
-  if (!rc) rc = upload(&h->dGidx, P.gidx.data(), P.gidx.size(), st);
-  if (!rc) rc = upload(&h->dGcols, P.gcols.data(), P.gcols.size(), st);
-  if (!rc) rc = upload(&h->dFblk, P.fblocks.data(), P.fblocks.size(), st);
-  if (!rc) rc = upload(&h->dFblkPtr, P.fblk_ptr.data(), P.fblk_ptr.size(), st);
-  if (!rc) rc = upload(&h->dSpanels, P.spanels.data(), P.spanels.size(), st);
-  if (!rc) rc = upload(&h->dSpOrder, P.sp_order.data(), P.sp_order.size(), st);
-  if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
-  if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
-  if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
   if (!rc) rc = upload(&h->dAmap, amap.data(), amap.size(), st);
   if (!rc) rc = upload(&h->dVals, (const double*)nullptr, (size_t)h->nvals, st);
   if (!rc) rc = upload(&h->dRp, frp.data(), frp.size(), st);
   if (!rc) rc = upload(&h->dCi, fci.data(), fci.size(), st);
   if (!rc) rc = upload(&h->dVmap, fvm.data(), fvm.size(), st);
   if (!rc) rc = upload(&h->dAf, (const double*)nullptr, fci.size(), st);
-  if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
-  if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);
   if (!rc) rc = upload(&h->dQi, qi.data(), qi.size(), st);
   if (!rc) rc = upload(&h->dQj, qj.data(), qj.size(), st);
   if (!rc) rc = upload(&h->dDiagPos, dpos.data(), dpos.size(), st);
-  h->zvalid = false;
-  (void)hipFree(h->dZ);
-  (void)hipFree(h->dArena);
-  (void)hipFree(h->dNodes);
-  h->dZ = nullptr;
-  h->dArena = nullptr;
-  h->dNodes = nullptr;
   // the copies above read this frame's vectors: wait for them on every path
   const hipError_t se = hipStreamSynchronize(st);
   if (rc) {
@@ -2308,6 +2378,7 @@ int swc_upload(swc_handle* h, int n, const int* blockSet, int supNo, const size_
     return rc;
   }
   SWC_HIP(se);
+  h->has_values = true;
   return SWC_OK;
 }
 
@@ -2356,7 +2427,7 @@ int swc_set_perm(swc_handle* h, const int* perm) {
 int swc_factor(swc_handle* h, const double* values, int values_on_device, int* info) {
   if (int rc = check_handle(h)) return rc;
   if (!values) return swc_set_error(SWC_EINVAL, "swc_factor: null values");
-  if (!h->dL) return swc_set_error(SWC_ESTATE, "swc_factor: no structure uploaded");
+  if (!h->dL || !h->has_values) return swc_set_error(SWC_ESTATE, "swc_factor: no structure uploaded");
   const double* dv = values;
   if (!values_on_device) {
     SWC_HIP(hipMemcpyAsync(h->dVals, values, sizeof(double) * h->nvals, hipMemcpyHostToDevice,
@@ -2597,6 +2668,84 @@ int sw_cholesky_left_par_05(int n, int* c, int* r, double* values, size_t* lC, i
     if (timing) timing[0] = h->t_factor_ms * 1e-3;  // device time of the factorization
   }
   return ok;
+}
+
+// Reference forward solve (triangularSolve/Triangular_BCSC.h:14-49, called as
+// examples/triangularTest02.cpp:195): L y = x in place on the host vector x, for
+// the BCSC factor (Lp = L->p, Li = L->s, Lx = values, Li_ptr = L->i_ptr, sup2col =
+// L->super, supNo = L->nsuper), on the current GPU.  NNZ and col2sup are not used
+// (the reference does not read them either).  The solve schedule comes from the
+// row lists (the call carries no descendant sets); it is built on the first call
+// for a structure and kept, with L's device buffer, for later calls with the
+// same structure (the arrays are compared, not their addresses).  Returns 1, or
+// 0 on failure (the reference's convention; swc_last_error() says why).
+namespace {
+std::mutex g_lsolve_mu;
+swc_handle* g_lsolve = nullptr;  // held until the process ends
+
+bool same_structure(const swc_handle* h, int n, const size_t* Lp, const int* Li,
+                    const size_t* Li_ptr, const int* sup2col, int supNo) {
+  if (!h || !h->dL || h->n != n || h->nsuper != supNo) return false;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev != h->device) return false;
+  if (std::memcmp(h->super.data(), sup2col, sizeof(int) * (size_t)(supNo + 1)) != 0) return false;
+  for (int s = 0; s <= supNo; ++s)
+    if ((int64_t)Li_ptr[sup2col[s]] != h->pi[s]) return false;
+  for (int s = 0; s < supNo; ++s)
+    if ((int64_t)Lp[sup2col[s]] != h->px[s]) return false;
+  return std::memcmp(h->Ls.data(), Li, sizeof(int) * h->Ls.size()) == 0;
+}
+}  // namespace
+
+int blockedLsolve(int n, size_t* Lp, int* Li, double* Lx, int NNZ, size_t* Li_ptr, int* col2sup,
+                  int* sup2col, int supNo, double* x) {
+  (void)NNZ; (void)col2sup;
+  if (!Lp || !Li || !x) return 0;  // the reference's own check
+  if (!Lx) {
+    swc_set_error(SWC_EINVAL, "blockedLsolve: null values");
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(g_lsolve_mu);
+  if (!same_structure(g_lsolve, n, Lp, Li, Li_ptr, sup2col, supNo)) {
+    if (validate_structure(n, sup2col, supNo, Lp, Li, Li_ptr)) return 0;
+    int dev = 0;
+    if (g_lsolve && (hipGetDevice(&dev) != hipSuccess || dev != g_lsolve->device)) {
+      delete g_lsolve;
+      g_lsolve = nullptr;
+    }
+    if (!g_lsolve && swc_create(&g_lsolve)) return 0;
+    std::vector<int> pp, ps;
+    std::vector<int64_t> pi(supNo + 1);
+    for (int s = 0; s <= supNo; ++s) pi[s] = (int64_t)Li_ptr[sup2col[s]];
+    try {
+      swc::descendant_sets(n, supNo, sup2col, pi.data(), Li, pp, ps);
+    } catch (const std::exception& e) {
+      swc_set_error(SWC_EINVAL, std::string("blockedLsolve: ") + e.what());
+      return 0;
+    }
+    if (upload_structure(g_lsolve, n, sup2col, supNo, Lp, Li, Li_ptr, pp.data(), ps.data(), false))
+      return 0;
+  }
+  swc_handle* h = g_lsolve;
+  if (ensure_work(h, 1)) return 0;
+  hipStream_t st = h->stream;
+  hipError_t e = hipMemcpyAsync(h->dL, Lx, sizeof(double) * (size_t)h->plan.xsize,
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h->dWork, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    enqueue_forward<1>(h, h->dWork, 1, n, h->dY, h->dV);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(x, h->dY, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st);
+  const hipError_t se = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = se;
+  if (e != hipSuccess) {
+    swc_set_error(SWC_EDEVICE, std::string("blockedLsolve: ") + hipGetErrorString(e));
+    return 0;
+  }
+  return 1;
 }
 
 }  // extern "C"

@@ -19,8 +19,45 @@ int upper_idx(const int* a, int len, int v) { return int(std::upper_bound(a, a +
 
 }  // namespace
 
+void descendant_sets(int n, int nsuper, const int* super, const int64_t* pi, const int* Ls,
+                     std::vector<int>& prunePtr, std::vector<int>& pruneSet) {
+  std::vector<int> col2sup(n);
+  for (int s = 0; s < nsuper; ++s)
+    for (int c = super[s]; c < super[s + 1]; ++c) col2sup[c] = s;
+  // (target, descendant) pairs, one per distinct target of each descendant's rows
+  std::vector<int> cnt(nsuper + 1, 0), last(nsuper, -1);
+  for (int d = 0; d < nsuper; ++d) {
+    const int W = super[d + 1] - super[d];
+    for (int64_t i = pi[d] + W; i < pi[d + 1]; ++i) {
+      const int r = Ls[i];
+      if (r < 0 || r >= n) throw std::invalid_argument("row index out of range");
+      const int t = col2sup[r];
+      if (t <= d) throw std::invalid_argument("row of a supernode below its own columns");
+      if (last[t] != d) {
+        last[t] = d;
+        cnt[t + 1]++;
+      }
+    }
+  }
+  for (int s = 0; s < nsuper; ++s) cnt[s + 1] += cnt[s];
+  prunePtr.assign(cnt.begin(), cnt.end());
+  pruneSet.assign(cnt[nsuper], 0);
+  std::fill(last.begin(), last.end(), -1);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int d = 0; d < nsuper; ++d) {
+    const int W = super[d + 1] - super[d];
+    for (int64_t i = pi[d] + W; i < pi[d + 1]; ++i) {
+      const int t = col2sup[Ls[i]];
+      if (last[t] != d) {
+        last[t] = d;
+        pruneSet[fill[t]++] = d;
+      }
+    }
+  }
+}
+
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
-                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P) {
+                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P, bool factor) {
   P = Plan();
   P.n = n;
   P.nsuper = nsuper;
@@ -56,7 +93,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   // ---- descendant sources of each panel ----
   struct Src { int d; int ca, cb; };  // d >= 0: supernode, d < 0: panel -(d+1) of same sn
   std::vector<std::vector<Src>> srcs(P.npanel);  // descendants touching each panel
-  for (int s = 0; s < nsuper; ++s) {
+  for (int s = 0; s < nsuper && factor; ++s) {
     const int fp = first_panel[s], lp = first_panel[s + 1];
     for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) {
       const int d = pruneSet[t];
@@ -173,7 +210,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     ptasks[ti].push_back(t);
   };
 
-  for (int s = 0; s < nsuper; ++s) {
+  for (int s = 0; s < nsuper && factor; ++s) {
     const int fp = first_panel[s], lp = first_panel[s + 1], np = lp - fp;
     int le = 0;
     for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) le = std::max(le, sn_level[pruneSet[t]] + 1);
@@ -461,15 +498,22 @@ extern "C" {
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out) {
-  // pruneSet may be null when it is empty (no supernode has descendants)
-  if (!out || !blockSet || !lC || !Ls || !Li_ptr || !prunePtr || supNo <= 0 ||
-      (!pruneSet && prunePtr[supNo] != 0))
+  // pruneSet may be null when it is empty (no supernode has descendants); with
+  // prunePtr null both are derived from the row lists (as blockedLsolve does)
+  if (!out || !blockSet || !lC || !Ls || !Li_ptr || supNo <= 0 ||
+      (prunePtr && !pruneSet && prunePtr[supNo] != 0))
     return swc_set_error(SWC_EINVAL, "swc_plan_build: bad argument");
   *out = nullptr;
   try {
     std::vector<int64_t> pi(supNo + 1), px(supNo);
     for (int s = 0; s <= supNo; ++s) pi[s] = (int64_t)Li_ptr[blockSet[s]];
     for (int s = 0; s < supNo; ++s) px[s] = (int64_t)lC[blockSet[s]];
+    std::vector<int> pp, ps;
+    if (!prunePtr) {
+      swc::descendant_sets(n, supNo, blockSet, pi.data(), Ls, pp, ps);
+      prunePtr = pp.data();
+      pruneSet = ps.data();
+    }
     auto* h = new swc_plan;
     swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P);
     *out = h;

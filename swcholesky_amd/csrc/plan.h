@@ -129,7 +129,16 @@ struct Plan {
 
 // super[nsuper+1], pi[nsuper+1] (row-list pointers), px[nsuper] value offsets,
 // Ls row indices, prunePtr/pruneSet as the reference's analysis produces them.
+// With factor = false only the solve part (solve panels, their levels, row
+// blocks and forward gathers) is built; the factor lists stay empty.
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
-                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P);
+                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P,
+                bool factor = true);
+
+// Descendant sets for a structure given without them: supernode s depends on
+// every d < s whose rows below its own columns meet s's columns, listed in
+// increasing d (the set the reference's pruneSet holds, in another order).
+void descendant_sets(int n, int nsuper, const int* super, const int64_t* pi, const int* Ls,
+                     std::vector<int>& prunePtr, std::vector<int>& pruneSet);
 
 }  // namespace swc

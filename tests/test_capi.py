@@ -33,3 +33,9 @@ def test_error_message_roundtrip():
     with pytest.raises(S.SwcError) as e:
         S.analyze_p2(3, np.array([0, 1, 2, 3]), np.array([0, 1, 2]), np.array([0, 0, 1]))
     assert "perm" in str(e.value)
+
+
+def test_blockedLsolve_null_inputs_return_zero():
+    """The reference returns 0 when Lp, Li or x is null (Triangular_BCSC.h:23)."""
+    lib = S.lib()
+    assert lib.blockedLsolve(1, None, None, None, 0, None, None, None, 1, None) == 0

@@ -232,3 +232,25 @@ def test_mixed_after_reupload_of_larger_structure():
     rhs = A @ np.random.default_rng(6).uniform(-1, 1, large.n)
     x, eta, _ = f.refine(rhs, tol=1e-12)
     assert eta[0] <= 1e-12 and O.backward_error(rp, ci, v, x, rhs) <= 1e-12
+
+
+@pytest.mark.parametrize("name,make", [SMALL[0], SMALL[3], SMALL[4], SMALL[6]],
+                         ids=[SMALL[0][0], SMALL[3][0], SMALL[4][0], SMALL[6][0]])
+def test_blockedLsolve_drop_in(name, make):
+    """The reference's forward-solve symbol, called with the arguments
+    triangularTest02.cpp:195 passes, against the oracle's forward solve, and the
+    closed form b = L 1 => y = 1 of rhsInitBlocked (Util.h:277-306).  A second call
+    on the same structure reuses the kept schedule."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    for rep in range(2):
+        x = np.random.default_rng(rep).uniform(-1, 1, b.n)
+        yo = O.lsolve(b, Lo, x.copy())[:, 0]
+        y = x.copy()
+        ok = S.blockedLsolve(b.n, b.p, b.s, Lo, b.nnz, b.i_ptr, b.col2Sup, b.super, b.nsuper, y)
+        assert ok == 1
+        assert np.linalg.norm(y - yo) / np.linalg.norm(yo) <= 1e-14
+    Ld = O.to_sparse_L(b, Lo)
+    y = Ld @ np.ones(b.n)
+    assert S.blockedLsolve(b.n, b.p, b.s, Lo, b.nnz, b.i_ptr, None, b.super, b.nsuper, y) == 1
+    assert np.max(np.abs(y - 1.0)) < 1e-12

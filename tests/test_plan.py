@@ -116,3 +116,24 @@ def test_emulated_solve_wide_supernodes(make):
     xe = E.solve(plan, b.s, Lo, y)
     xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
     assert relF(xe, xo) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["c1_lap2d_const_100", "gmrf2d_32", "lap3d_var_12", "vec3d_var_6"])
+def test_derived_descendant_sets_give_the_reference_solve_schedule(name):
+    """blockedLsolve gets no descendant sets and derives them from the row lists.
+    With the reference analysis's own structure (golden fixtures written by its
+    analyze_p2), the solve schedule built from the derived sets equals the one built
+    from the reference's pruneSet."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    prob = {"c1_lap2d_const_100": lambda: P.laplacian2d_const(100),
+            "gmrf2d_32": lambda: P.gmrf2d(32), "lap3d_var_12": lambda: P.laplacian3d_var(12),
+            "vec3d_var_6": lambda: P.vector3d_var(6)}[name]()
+    b, *_ = setup(prob)
+    assert np.array_equal(g["pruneset"], b.pruneSet) and np.array_equal(g["s"], b.s)
+    ref = api.build_plan(b)
+    der = api.build_plan(b, derive_sets=True)
+    for key in ["spanels", "sp_order", "lvl_sp", "fblocks", "lvl_fblk", "gptr", "gidx", "gcols",
+                "lvl_gcol"]:
+        assert np.array_equal(ref[key], der[key]), key
+    assert ref["nslevel"] == der["nslevel"]
