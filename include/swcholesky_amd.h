@@ -196,7 +196,8 @@ int swc_get_timings(const swc_handle* h, double* out);
  * level->solve-panel offsets) as the packed structs and arrays of
  * swcholesky_amd/csrc/plan.h.  With prunePtr NULL the descendant sets are derived
  * from the row lists (supernode s depends on every d whose rows below its own
- * columns meet s's columns), as blockedLsolve derives them. */
+ * columns meet s's columns) and only the solve part of the plan is built, as
+ * blockedLsolve builds it (the factor lists stay empty). */
 typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,

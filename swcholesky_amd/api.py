@@ -20,6 +20,11 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr
 
+# Release functions bound once, so that finalizers that run while the interpreter
+# shuts down (module globals already cleared) still release their handles.
+_FREE = _lib.lib().swc_symbolic_free
+_DESTROY = _lib.lib().swc_destroy
+
 _SYM = {
     "Perm": (0, "n", np.int32), "ColCount": (1, "n", np.int32), "parent": (2, "n", np.int32),
     "super": (3, "ns1", np.int32), "sParent": (4, "ns", np.int32), "col2Sup": (5, "n", np.int32),
@@ -47,9 +52,9 @@ class BCSC:
             setattr(self, name, a)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().swc_symbolic_free(self._h)
-            self._h = None
+        h, self._h = getattr(self, "_h", None), None
+        if h and _FREE is not None:
+            _FREE(h)
 
     def a2_values(self, values):
         """Values of tril(P A P^T) in A2 order from values of A in input order."""
@@ -156,9 +161,9 @@ class Factor:
         self.info = None
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().swc_destroy(self._h)
-            self._h = None
+        h, self._h = getattr(self, "_h", None), None
+        if h and _DESTROY is not None:
+            _DESTROY(h)
 
     def factor(self, values):
         """Numeric factorization; returns LAPACK-style info (0 = success)."""

@@ -253,12 +253,12 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   P.nlevel = 0;
   for (const PTile& pt : ptiles) P.nlevel = std::max(P.nlevel, pt.level + 1);
 
-  // ---- panels grouped by level (forward solve order) ----
+  // ---- panels grouped by factor level (a solve-only plan has no factor levels) ----
   P.lvl_panels.assign(P.nlevel + 1, 0);
-  for (int q = 0; q < P.npanel; ++q) P.lvl_panels[P.panel_level[q] + 1]++;
-  for (int l = 0; l < P.nlevel; ++l) P.lvl_panels[l + 1] += P.lvl_panels[l];
-  P.panel_order.resize(P.npanel);
-  {
+  if (factor) {
+    for (int q = 0; q < P.npanel; ++q) P.lvl_panels[P.panel_level[q] + 1]++;
+    for (int l = 0; l < P.nlevel; ++l) P.lvl_panels[l + 1] += P.lvl_panels[l];
+    P.panel_order.resize(P.npanel);
     std::vector<int> fill(P.lvl_panels.begin(), P.lvl_panels.end() - 1);
     for (int q = 0; q < P.npanel; ++q) P.panel_order[fill[P.panel_level[q]]++] = q;
   }
@@ -499,7 +499,8 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out) {
   // pruneSet may be null when it is empty (no supernode has descendants); with
-  // prunePtr null both are derived from the row lists (as blockedLsolve does)
+  // prunePtr null both are derived from the row lists and only the solve part of
+  // the plan is built (what blockedLsolve does)
   if (!out || !blockSet || !lC || !Ls || !Li_ptr || supNo <= 0 ||
       (prunePtr && !pruneSet && prunePtr[supNo] != 0))
     return swc_set_error(SWC_EINVAL, "swc_plan_build: bad argument");
@@ -509,13 +510,14 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
     for (int s = 0; s <= supNo; ++s) pi[s] = (int64_t)Li_ptr[blockSet[s]];
     for (int s = 0; s < supNo; ++s) px[s] = (int64_t)lC[blockSet[s]];
     std::vector<int> pp, ps;
-    if (!prunePtr) {
+    const bool factor = prunePtr != nullptr;
+    if (!factor) {
       swc::descendant_sets(n, supNo, blockSet, pi.data(), Ls, pp, ps);
       prunePtr = pp.data();
       pruneSet = ps.data();
     }
     auto* h = new swc_plan;
-    swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P);
+    swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P, factor);
     *out = h;
     return SWC_OK;
   } catch (const std::exception& e) {

@@ -123,7 +123,9 @@ def test_derived_descendant_sets_give_the_reference_solve_schedule(name):
     """blockedLsolve gets no descendant sets and derives them from the row lists.
     With the reference analysis's own structure (golden fixtures written by its
     analyze_p2), the solve schedule built from the derived sets equals the one built
-    from the reference's pruneSet."""
+    from the reference's pruneSet.  The derived plan is the solve-only plan
+    blockedLsolve uploads (no factor levels), and a solve emulated on it matches
+    the oracle's."""
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
     prob = {"c1_lap2d_const_100": lambda: P.laplacian2d_const(100),
@@ -137,3 +139,9 @@ def test_derived_descendant_sets_give_the_reference_solve_schedule(name):
                 "lvl_gcol"]:
         assert np.array_equal(ref[key], der[key]), key
     assert ref["nslevel"] == der["nslevel"]
+    assert der["nlevel"] == 0 and len(der["tiles"]) == 0 and len(der["tasks"]) == 0
+    _, a2x, Lo, _ = setup(prob)
+    y = np.random.default_rng(2).uniform(-1, 1, b.n)
+    xe = E.solve(der, b.s, Lo, y)
+    xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
+    assert relF(xe, xo) < 1e-12
