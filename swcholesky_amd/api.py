@@ -20,11 +20,6 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr
 
-# Release functions bound once, so that finalizers that run while the interpreter
-# shuts down (module globals already cleared) still release their handles.
-_FREE = _lib.lib().swc_symbolic_free
-_DESTROY = _lib.lib().swc_destroy
-
 _SYM = {
     "Perm": (0, "n", np.int32), "ColCount": (1, "n", np.int32), "parent": (2, "n", np.int32),
     "super": (3, "ns1", np.int32), "sParent": (4, "ns", np.int32), "col2Sup": (5, "n", np.int32),
@@ -40,6 +35,9 @@ class BCSC:
 
     def __init__(self, handle, sizes, flops):
         self._h = handle
+        # kept on the instance: a finalizer run while the interpreter shuts down
+        # (module globals already cleared) still releases the handle
+        self._free = _lib.lib().swc_symbolic_free
         self.n, self.nsuper, self.ssize, self.xsize, self.maxSupWid, self.maxCol, self.nnz, \
             self.nprune = (int(v) for v in sizes)
         self.flops = float(flops)  # sum_j ColCount[j]^2
@@ -53,8 +51,8 @@ class BCSC:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and _FREE is not None:
-            _FREE(h)
+        if h:
+            self._free(h)
 
     def a2_values(self, values):
         """Values of tril(P A P^T) in A2 order from values of A in input order."""
@@ -156,14 +154,15 @@ class Factor:
         lib = _lib.lib()
         h = ctypes.c_void_p()
         check(lib.swc_create(ctypes.byref(h)))
+        self._destroy = lib.swc_destroy  # see BCSC
         self._h = h
         check(lib.swc_upload_symbolic(self._h, bcsc._h))
         self.info = None
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and _DESTROY is not None:
-            _DESTROY(h)
+        if h:
+            self._destroy(h)
 
     def factor(self, values):
         """Numeric factorization; returns LAPACK-style info (0 = success)."""

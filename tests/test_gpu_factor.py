@@ -240,7 +240,10 @@ def test_blockedLsolve_drop_in(name, make):
     """The reference's forward-solve symbol, called with the arguments
     triangularTest02.cpp:195 passes, against the oracle's forward solve, and the
     closed form b = L 1 => y = 1 of rhsInitBlocked (Util.h:277-306).  A second call
-    on the same structure reuses the kept schedule."""
+    on the same structure reuses the kept schedule.  The closed form's forward
+    error grows with the conditioning of L (the oracle itself is 6e-12 from 1 on
+    lap3d_var_20), so it is held to the oracle's own error, and the residual
+    ||L y - b|| / (||L|| ||y||) to 1e-14 (the oracle reaches 1.3e-15)."""
     prob = make()
     b, a2x, Lo, _ = setup(prob)
     for rep in range(2):
@@ -251,6 +254,10 @@ def test_blockedLsolve_drop_in(name, make):
         assert ok == 1
         assert np.linalg.norm(y - yo) / np.linalg.norm(yo) <= 1e-14
     Ld = O.to_sparse_L(b, Lo)
-    y = Ld @ np.ones(b.n)
+    rhs = Ld @ np.ones(b.n)
+    err_o = np.max(np.abs(O.lsolve(b, Lo, rhs.copy())[:, 0] - 1.0))
+    y = rhs.copy()
     assert S.blockedLsolve(b.n, b.p, b.s, Lo, b.nnz, b.i_ptr, None, b.super, b.nsuper, y) == 1
-    assert np.max(np.abs(y - 1.0)) < 1e-12
+    assert np.max(np.abs(y - 1.0)) <= 4 * err_o + 1e-14
+    res = np.max(np.abs(Ld @ y - rhs)) / (abs(Ld).sum(axis=1).max() * np.max(np.abs(y)))
+    assert res <= 1e-14
