@@ -138,6 +138,15 @@ int swc_get_factor(swc_handle* h, double* lValues, int on_device);
  * permutation, forward L y = P b (blockedLsolve, triangularSolve/Triangular_BCSC.h:14-49),
  * backward L^T z = y (no reference counterpart), inverse permutation. */
 int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): swc_solve on device columns (B on the current device,
+ * ldb >= n) that calls no hipMalloc/hipFree and captures no graph.  Its
+ * workspace is a fixed 128 MiB device array per GPU, held until the process
+ * ends; a call that needs more (about (2n + vsize + 128 max_lvl_fblk) * 8 bytes
+ * per right-hand side) fails with SWC_EINVAL.  Calls on one GPU share that
+ * array: calls that overlap in time on one GPU race. */
+int swc_solve_noalloc(swc_handle* h, double* B, int nrhs, int64_t ldb);
 /* Solves A X = B to backward error eta = ||b - A x||_inf / (||A||_inf ||x||_inf +
  * ||b||_inf) <= tol per column, by FP64 refinement on the current factor: the
  * residual is an FP64 SpMV with the values given to swc_factor, corrections use

@@ -39,6 +39,7 @@ SIGNATURES = {
     "swc_factor": (c_int, [c_vp, c_vp, c_int, c_ip]),
     "swc_get_factor": (c_int, [c_vp, c_vp, c_int]),
     "swc_solve": (c_int, [c_vp, c_vp, c_int, c_i64, c_int]),
+    "swc_solve_noalloc": (c_int, [c_vp, c_vp, c_int, c_i64]),
     "swc_plan_stats": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_level_timings": (c_int, [c_vp, c_vp, c_int, c_vp]),
     "swc_set_profiling": (c_int, [c_vp, c_int]),

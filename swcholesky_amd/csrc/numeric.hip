@@ -1108,6 +1108,36 @@ __global__ void k_permute(const double* __restrict__ in, double* __restrict__ ou
 
 constexpr int kRedBlocks = 512;
 
+// Workspace of swc_solve_noalloc, the entry point that allocates nothing: one
+// copy of this array per GPU, held from the library's load until the process
+// ends.  Host code never uses its name as an address: swc_create looks up each
+// device's copy once (scratch_ptr) and everything else uses that pointer.
+constexpr int64_t kScratchDoubles = (int64_t)16 << 20;  // 128 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Per-device address of `scratch`, looked up once per device with that device
+// current (hipGetSymbolAddress); null until then.
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;
+
+int bind_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return SWC_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return swc_set_error(SWC_EDEVICE, std::string("no-alloc workspace: hipGetSymbolAddress: ") +
+                                          hipGetErrorString(e));
+  g_scratch_ptr[device] = (double*)ptr;
+  return SWC_OK;
+}
+
+double* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
 // Af[e] = vals[vmap[e]]: the full P A P^T values in CSR order, from the caller's values.
 __global__ void k_gather(double* __restrict__ out, const double* __restrict__ vals,
                          const int64_t* __restrict__ vmap, int64_t nnz) {
@@ -1852,9 +1882,10 @@ void enqueue_backward(swc_handle* h, double* dX, int nrhs, int64_t ldx, const do
 }
 
 template <int RF, int RP, int RD>
-void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  enqueue_forward<RF>(h, dX, nrhs, ldx, h->dY, h->dV);
-  enqueue_backward<RP, RD>(h, dX, nrhs, ldx, h->dY, h->dPb);
+void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx, double* dY, double* dV,
+                      double* dPb) {
+  enqueue_forward<RF>(h, dX, nrhs, ldx, dY, dV);
+  enqueue_backward<RP, RD>(h, dX, nrhs, ldx, dY, dPb);
 }
 
 // Groups of right-hand sides: each block of L is read once per group, while
@@ -1863,10 +1894,17 @@ void enqueue_solve_rb(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
 // wave per SIMD); the backward partials, which have no chain, take 8, or 16
 // from 16 right-hand sides on; the backward diagonal, mostly chain, takes one
 // per workgroup (measured on the vector 48^3 operator at 8 and 32 RHS).
+// dY (n x nrhs), dV (max(vsize, 1) x nrhs) and dPb (max(max_lvl_fblk, 1) SPW x
+// nrhs) are the sweeps' workspace.
+void enqueue_solve_bufs(swc_handle* h, double* dX, int nrhs, int64_t ldx, double* dY, double* dV,
+                        double* dPb) {
+  if (nrhs >= 16) enqueue_solve_rb<4, 16, 1>(h, dX, nrhs, ldx, dY, dV, dPb);
+  else if (nrhs >= 2) enqueue_solve_rb<4, 8, 1>(h, dX, nrhs, ldx, dY, dV, dPb);
+  else enqueue_solve_rb<1, 1, 1>(h, dX, nrhs, ldx, dY, dV, dPb);
+}
+
 void enqueue_solve(swc_handle* h, double* dX, int nrhs, int64_t ldx) {
-  if (nrhs >= 16) enqueue_solve_rb<4, 16, 1>(h, dX, nrhs, ldx);
-  else if (nrhs >= 2) enqueue_solve_rb<4, 8, 1>(h, dX, nrhs, ldx);
-  else enqueue_solve_rb<1, 1, 1>(h, dX, nrhs, ldx);
+  enqueue_solve_bufs(h, dX, nrhs, ldx, h->dY, h->dV, h->dPb);
 }
 
 // In-place solve of L L^T X = B on device columns X (factor ordering); needs
@@ -2284,6 +2322,10 @@ int swc_create(swc_handle** out) {
   auto* h = new (std::nothrow) swc_handle;
   if (!h) return swc_set_error(SWC_ENOMEM, "swc_create: out of host memory");
   hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess && bind_scratch(h->device) != SWC_OK) {
+    delete h;
+    return SWC_EDEVICE;  // bind_scratch recorded why
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&h->dInfo, sizeof(int));
   for (auto& ev : h->ev)
@@ -2462,6 +2504,46 @@ int swc_solve(swc_handle* h, double* B, int nrhs, int64_t ldb, int on_device) {
   if (int rc = to_factor(h, B, on_device, ldb, dY, nrhs)) return rc;
   if (int rc = solve_device(h, dY, nrhs, h->n)) return rc;
   if (int rc = from_factor(h, dY, B, on_device, ldb, nrhs)) return rc;
+  float ms = 0;
+  SWC_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+  h->t_solve_ms = ms;
+  return SWC_OK;
+}
+
+// swc_solve on device columns with every buffer the sweeps need taken from the
+// static `scratch` array (through this device's looked-up address): no device
+// allocation, no captured graph (eager launches on the handle's stream).
+// Layout, each part starting on a 256-byte boundary: the permuted right-hand
+// sides (n x nrhs), the forward solutions (n x nrhs), the contribution vectors
+// (max(vsize,1) x nrhs, cleared by the forward sweep's fill kernel) and the
+// backward partials.  Calls on one GPU share the array: overlapping calls race.
+int swc_solve_noalloc(swc_handle* h, double* B, int nrhs, int64_t ldb) {
+  if (int rc = check_handle(h)) return rc;
+  if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_solve_noalloc: not factored");
+  if (h->last_info) return swc_set_error(SWC_ENOTPOS, "swc_solve_noalloc: factorization failed");
+  if (!B || nrhs <= 0 || nrhs > 65535 || ldb < h->n)
+    return swc_set_error(SWC_EINVAL, "swc_solve_noalloc: bad argument");
+  double* base = scratch_ptr(h->device);
+  if (!base) return swc_set_error(SWC_ESTATE, "swc_solve_noalloc: workspace not bound to this device");
+  const Plan& P = h->plan;
+  auto up = [](int64_t x) { return (x + 31) & ~(int64_t)31; };  // 32 doubles = 256 bytes
+  const int64_t n = h->n;
+  const int64_t vld = std::max<int64_t>(P.vsize, 1);
+  const int64_t pld = (int64_t)std::max(P.max_lvl_fblk, 1) * swc::SPW;
+  const int64_t oX = 0, oY = oX + up(n * nrhs), oV = oY + up(n * nrhs), oP = oV + up(vld * nrhs);
+  const int64_t need = oP + up(pld * nrhs);
+  if (need > kScratchDoubles)
+    return swc_set_error(SWC_EINVAL, "swc_solve_noalloc: needs " + std::to_string(need * 8) +
+                                         " bytes of workspace for " + std::to_string(nrhs) +
+                                         " right-hand sides; the fixed workspace holds " +
+                                         std::to_string(kScratchDoubles * 8));
+  double* Xt = base + oX;
+  if (int rc = to_factor(h, B, 1, ldb, Xt, nrhs)) return rc;
+  SWC_HIP(hipEventRecord(h->ev[2], h->stream));
+  enqueue_solve_bufs(h, Xt, nrhs, n, base + oY, base + oV, base + oP);
+  SWC_HIP(hipEventRecord(h->ev[3], h->stream));
+  SWC_HIP(hipGetLastError());
+  if (int rc = from_factor(h, Xt, B, 1, ldb, nrhs)) return rc;
   float ms = 0;
   SWC_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
   h->t_solve_ms = ms;
