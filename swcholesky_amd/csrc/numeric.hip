@@ -193,7 +193,7 @@ __device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
 // dimension over the whole 64x64 tile in MFMA registers; the four partial sums
 // are combined in a fixed order, (w0 + w1) + (w2 + w3), through LDS and
 // subtracted from the tile, so results do not depend on scheduling.  The
-// factorization of panels completed by this level follows in k_panel_factor.
+// factorization of panels completed by this level follows in the panel factor kernels.
 // MODE 0: FP64 throughout; 1: descendant (kind 0) tiles on the FP32 copy,
 // intra-supernode tiles in FP64 (a tile's tasks all have one kind); 2: FP32
 // for every tile.
@@ -404,122 +404,177 @@ k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
 }
 
 
-// Panel factorization of the panels completed at this level, one workgroup of
-// NW waves per tile, for panels of at most 16 NW columns: POTRF of the panel's
-// nc x nc diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430), then
-// B <- B L11^{-T} for the tile's rows below the diagonal block (dtrsm
-// 'R','L','T','N', .c:461-467).  Every tile of a panel computes the same L11
-// from the same launch-invariant copy (bitwise identical), so no tile waits
-// for another; the diagonal tile stores it and reports info.
-//
-// Wave w holds columns [16w, 16w+16) of row `lane` of the diagonal block
-// (padded to 16 NW with the identity) and of the tile's rows (padded with
-// zeros).  Both go block column by block column: in phase J wave J factors its
-// 16 columns, publishing each column of L11 in LDS as it completes; the other
-// rows of the block column read its entries from there (broadcast reads), and
-// only the entry the next pivot depends on is read from its lane.  Wave J then
-// substitutes its 16 columns of the tile rows (scaling by the pivots'
-// reciprocal square roots) and publishes them; the waves to its right
-// subtract the block's products from their columns of both, in increasing
-// column order, as the left-looking loops of the reference do.
-template <int NW>
-__global__ void __launch_bounds__(64 * NW)
-k_panel_factor(double* __restrict__ L, const Panel* __restrict__ panels,
-               const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
-               const double* __restrict__ dscratch, int* __restrict__ info_out,
-               float* __restrict__ Lf) {
-  static_assert(NW >= 2, "panels of at most 16 columns use k_panel_factor_regs");
-  constexpr int NCOL = 16 * NW;
-  __shared__ double sl[NCOL * kLdsLd];  // L11: column c at sl[c * kLdsLd]
-  __shared__ double sx[NCOL * kLdsLd];  // solved tile columns, same layout
-  __shared__ double sinv[NCOL];         // 1 / L11(c, c)
+// Column stride (doubles) of the LDS working matrix of k_panel_factor_blk: rows
+// 0..63 hold the panel's diagonal block, rows 64..127 the tile rows being solved.
+// An odd stride spreads the 16 columns an MFMA output block touches over all
+// LDS banks.
+constexpr int kWld = 2 * swc::TM + 1;
+
+// acc (16x16 block of the f64 MFMA output layout: lane l holds rows l/16 + 4r,
+// column l%16) -= A B^T over 16 columns, A = W rows [16 ra, 16 ra + 16),
+// B = W rows [16 rb, 16 rb + 16), both from columns [kb, kb + 16) of W.  The
+// MFMA accumulates its products one after another, as a chain of FMAs would.
+__device__ inline d4 lds_block_update(const double* __restrict__ W, d4 acc, int ra, int rb, int kb,
+                                      int lane) {
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const double* col = W + (kb + lk + 4 * u) * kWld;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-col[16 * ra + lr], col[16 * rb + lr], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Panel factorization of the panels completed at this level for panels of
+// 17..64 columns, one workgroup of four waves per tile: POTRF of the panel's
+// nc x nc diagonal block (dpotrf 'L', sw_parallel_PB_Cholesky_05.c:430) and
+// B <- B L11^{-T} for the tile's rows below it (dtrsm 'R','L','T','N',
+// .c:461-467), blocked in 16 columns and right-looking, as the reference's
+// dpotrf_cluster_master blocks its POTRF (BLAS-sw/dpotrf_cluster_master.c:11-63).
+// The diagonal block (from the launch-invariant copy, identity past nc) and the
+// tile rows sit in LDS (W) from the first load to the final store.  Per block
+// column J:
+//  (1) one sweep over the 16 columns factors the 16x16 diagonal block and
+//      solves every row below it at once, lane = row: wave 0 takes the diagonal
+//      block's rows, waves 1 and 2 the tile rows, each of them also carrying
+//      the 16 pivot rows in lanes 0..15 (the same operations on the same
+//      values, so all three compute the same pivots bitwise).  Each pivot's
+//      reciprocal square root scales its column; the next pivot is read from
+//      its lane (readlane) and the later columns' multipliers from LDS, so the
+//      pivot chain does not wait for LDS;
+//  (2) the later block columns of the diagonal block (blocks on or below its
+//      diagonal) and of the tile rows take the rank-16 update on FP64 MFMA,
+//      the 16x16 output blocks spread over the four waves.
+// Both apply each entry's updates in increasing column order, so the result is
+// bitwise the one of a column-by-column right-looking factorization.  Every
+// tile of a panel computes the same L11, so no tile waits for another; the
+// diagonal tile stores it and reports info.
+__global__ void __launch_bounds__(256)
+k_panel_factor_blk(double* __restrict__ L, const Panel* __restrict__ panels,
+                   const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
+                   const double* __restrict__ dscratch, int* __restrict__ info_out,
+                   float* __restrict__ Lf) {
+  __shared__ double W[swc::PW * kWld];
+  __shared__ double lc[3][16][16];  // per sweeping wave: column j's multipliers
   const Tile tl = tiles[list[list0 + blockIdx.x]];
   const Panel pn = panels[tl.panel];
-  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nr = tl.nr, nc = pn.ncol;
-  const int c0 = 16 * w;  // first column of this wave
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nc = pn.ncol;
+  const int nb = (nc + 15) >> 4;              // 16-column blocks (2..4)
+  const int first = tl.kind == 0 ? nc : 0;    // tile rows [first, nr) are solved ...
+  const int nx = tl.nr > first ? tl.nr - first : 0;  // ... into W rows 64 .. 64 + nx
+  const int nxb = (nx + 15) >> 4;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
-  // the tile's rows below the diagonal block are tile rows [first, nr)
-  const int first = tl.kind == 0 ? nc : 0;
-  const bool mine = lane >= first && lane < nr;
   double* gtile = L + pn.vbase + tl.r0;
+  {
+    // wave w moves columns 16w..16w+15 of diagonal-block row `lane` and of tile
+    // row `lane`; every lane loads from an address inside both buffers and then
+    // selects, so all its loads are in flight together
+    const int c0 = 16 * wave;
+    double y[16], x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int col = c0 + c;
+      const bool iny = lane < nc && col < nc && col <= lane, inx = lane < nx && col < nc;
+      const double vy = gdiag[iny ? col * nc + lane : 0];
+      const double vx = gtile[inx ? (int64_t)col * pn.ld + first + lane : 0];
+      y[c] = iny ? vy : ((lane >= nc || col >= nc) && col == lane ? 1.0 : 0.0);
+      x[c] = inx ? vx : 0.0;
+    }
+    if (c0 < 16 * nb) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        W[(c0 + c) * kWld + lane] = y[c];
+        W[(c0 + c) * kWld + 64 + lane] = x[c];
+      }
+    }
+  }
+  __syncthreads();
+  int badcol = 0;
+  for (int J = 0; J < nb; ++J) {
+    const int jb = 16 * J;
+    // (1) this lane's W row (wave 3 only reads one); the lane holding pivot row
+    // jb + j is pbase + j.  Every wave has read its rows before any is written
+    // back (waves 1 and 2 read the pivot rows that wave 0 rewrites).
+    const int row = wave == 0 ? lane
+                              : (wave == 3 || lane < 16 || (wave == 2 && lane >= 32)
+                                     ? jb + (lane & 15)
+                                     : 64 + 48 * (wave - 1) + (lane - 16));
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[c] = W[(jb + c) * kWld + row];
+    __syncthreads();
+    if (wave < 3) {
+      const int pbase = wave == 0 ? jb : 0;
+      const bool store = wave == 0 ? (lane >= jb && lane < 16 * nb)
+                                   : (lane >= 16 && row - 64 < nx && (wave == 1 || lane < 32));
+      double(*mult)[16] = lc[wave];
+      double d = bcast(x[0], pbase);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (d > 0.0) {
+          // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
+          // by 1/ajj)
+          x[j] *= rsqrt(d);
+        } else {
+          // a non-positive pivot is reported and continues as 1, so no NaNs spread
+          badcol = badcol == 0 ? jb + j + 1 : badcol;
+          x[j] = lane == pbase + j ? 1.0 : x[j];
+        }
+        if (j < 15) {
+          // the next pivot is its lane's x[j+1] after this column's update
+          d = bcast(x[j + 1] - x[j] * x[j], pbase + j + 1);
+          if (lane >= pbase && lane < pbase + 16) mult[j][lane - pbase] = x[j];
+#pragma unroll
+          for (int c = j + 1; c < 16; ++c) x[c] -= x[j] * mult[j][c];
+        }
+      }
+      if (store) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) W[(jb + c) * kWld + row] = x[c];
+      }
+    }
+    __syncthreads();
+    if (J + 1 < nb) {
+      // (2) block columns C > J: rows of the diagonal block from C on, then the
+      // tile rows (W row blocks 4 .. 4 + nxb)
+      int idx = 0;
+      for (int C = J + 1; C < nb; ++C) {
+        const int nrb = (nb - C) + nxb;
+        for (int q = 0; q < nrb; ++q, ++idx) {
+          if ((idx & 3) != wave) continue;
+          const int R = q < nb - C ? C + q : 4 + (q - (nb - C));
+          double* out = W + (16 * C + (lane & 15)) * kWld + 16 * R + (lane >> 4);
+          d4 acc = d4{out[0], out[4], out[8], out[12]};
+          acc = lds_block_update(W, acc, R, C, jb, lane);
+          out[0] = acc[0];
+          out[4] = acc[1];
+          out[8] = acc[2];
+          out[12] = acc[3];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tl.kind == 0 && badcol && badcol <= nc && tid == 0) atomicMin(info_out, pn.c0 + badcol);
   float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
-  // every lane loads (entries outside the block or the tile read element 0,
-  // which both have), then selects: no branches around the loads, so the
-  // tile's latency overlaps the diagonal block's
-  double y[16], x[16];
+  const int c0 = 16 * wave;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
     const int col = c0 + c;
-    const bool iny = lane < nc && col < nc && col <= lane, inx = mine && col < nc;
-    const double vy = gdiag[iny ? col * nc + lane : 0];
-    const double vx = gtile[inx ? (int64_t)col * pn.ld + lane : 0];
-    y[c] = iny ? vy : ((lane >= nc || col >= nc) && col == lane ? 1.0 : 0.0);
-    x[c] = inx ? vx : 0.0;
-  }
-  int badcol = 0;
-  for (int J = 0; J < NW; ++J) {
-    if (w == J) {
-      double* colj = sl + c0 * kLdsLd;  // column c0 + jj at colj[jj * kLdsLd]
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const int jg = c0 + jj;
-        const double d = bcast(y[jj], jg);
-        const bool ok = d > 0.0;
-        badcol = (!ok && badcol == 0) ? jg + 1 : badcol;
-        // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
-        // by 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
-        const double rs = ok ? rsqrt(d) : 1.0;
-        y[jj] = lane == jg ? (ok ? d * rs : 1.0) : (lane > jg ? y[jj] * rs : y[jj]);
-        if (lane == jg) sinv[jg] = rs;
-        colj[jj * kLdsLd + lane] = y[jj];
-        if (jj < 15) {
-          y[jj + 1] -= y[jj] * bcast(y[jj], jg + 1);
-#pragma unroll
-          for (int c = jj + 2; c < 16; ++c) y[c] -= y[jj] * colj[jj * kLdsLd + c0 + c];
-        }
+    if (col < nc) {
+      if (tl.kind == 0 && lane < nc && col <= lane) {
+        const double v = W[col * kWld + lane];
+        gtile[(int64_t)col * pn.ld + lane] = v;
+        if (ftile) ftile[(int64_t)col * pn.ld + lane] = (float)v;
       }
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        x[jj] *= sinv[c0 + jj];
-        const double* lj = colj + jj * kLdsLd + c0;
-#pragma unroll
-        for (int c = jj + 1; c < 16; ++c) x[c] -= x[jj] * lj[c];
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) sx[(c0 + c) * kLdsLd + lane] = x[c];
-    }
-    __syncthreads();
-    if (w > J) {
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const double* lj = sl + (16 * J + jj) * kLdsLd;
-        const double lij = lj[lane];
-        const double xj = sx[(16 * J + jj) * kLdsLd + lane];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          const double lcj = lj[c0 + c];
-          y[c] -= lij * lcj;
-          x[c] -= xj * lcj;
-        }
+      if (lane < nx) {
+        const double v = W[col * kWld + 64 + lane];
+        gtile[(int64_t)col * pn.ld + first + lane] = v;
+        if (ftile) ftile[(int64_t)col * pn.ld + first + lane] = (float)v;
       }
     }
   }
-  if (tl.kind == 0) {
-    if (badcol && badcol <= nc && lane == 0) atomicMin(info_out, pn.c0 + badcol);
-#pragma unroll
-    for (int c = 0; c < 16; ++c)
-      if (lane < nc && c0 + c < nc && c0 + c <= lane) {
-        gtile[(int64_t)(c0 + c) * pn.ld + lane] = y[c];
-        if (ftile) ftile[(int64_t)(c0 + c) * pn.ld + lane] = (float)y[c];
-      }
-  }
-#pragma unroll
-  for (int c = 0; c < 16; ++c)
-    if (mine && c0 + c < nc) {
-      gtile[(int64_t)(c0 + c) * pn.ld + lane] = x[c];
-      if (ftile) ftile[(int64_t)(c0 + c) * pn.ld + lane] = (float)x[c];
-    }
 }
 
 // ---- triangular solves --------------------------------------------------
@@ -1737,12 +1792,9 @@ void enqueue_factor(swc_handle* h, bool profile) {
       if (cls[1] > cls[0])
         hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
                            lp, cls[0], (const double*)h->dDiag, h->dInfo, lf);
-      if (cls[2] > cls[1])
-        hipLaunchKernelGGL(k_panel_factor<2>, dim3(cls[2] - cls[1]), dim3(128), 0, st, h->dL, pp, tp,
-                           lp, cls[1], (const double*)h->dDiag, h->dInfo, lf);
-      if (cls[3] > cls[2])
-        hipLaunchKernelGGL(k_panel_factor<4>, dim3(cls[3] - cls[2]), dim3(256), 0, st, h->dL, pp,
-                           tp, lp, cls[2], (const double*)h->dDiag, h->dInfo, lf);
+      if (cls[3] > cls[1])
+        hipLaunchKernelGGL(k_panel_factor_blk, dim3(cls[3] - cls[1]), dim3(256), 0, st, h->dL, pp,
+                           tp, lp, cls[1], (const double*)h->dDiag, h->dInfo, lf);
     }
     if (pe) (void)hipEventRecord(pe[2], st);
   }

@@ -71,27 +71,149 @@ def factor(plan, Ls, xsize, amap, vals):
             nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
             didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
             a11 = scratch[int(tl["slot"])].copy()  # the launch-invariant copy
-            l11 = np.zeros_like(a11)
-            for j in range(nc):  # right-looking, as the kernel does
-                d = a11[j, j]
-                if not d > 0:
-                    if tl["kind"] == 0:
-                        info = info or int(pn["c0"]) + j + 1
-                    d = 1.0
-                l11[j, j] = np.sqrt(d) if a11[j, j] > 0 else 1.0
-                l11[j + 1:, j] = a11[j + 1:, j] / l11[j, j]
-                for c in range(j + 1, nc):
-                    a11[c:, c] -= l11[c:, j] * l11[c, j]
             first = nc if tl["kind"] == 0 else 0
-            if tl["kind"] == 0:
-                keep = np.tril(np.ones((nc, nc), bool))
-                L[didx[keep]] = l11[keep]
-            if nr > first:
-                base = int(pn["vbase"]) + int(tl["r0"])
-                idx = base + np.arange(nc)[None, :] * ld + np.arange(first, nr)[:, None]
-                _chk(int(idx.min()), int(idx.max()) + 1, xsize, "factor tile")
-                L[idx] = np.linalg.solve(l11, L[idx].T).T
+            base = int(pn["vbase"]) + int(tl["r0"])
+            if nc > 16:
+                bad = panel_factor_blk(L, base, ld, a11, nc, nr, first, xsize)
+            else:
+                bad = _panel_factor_scalar(L, base, ld, a11, nc, nr, first, xsize)
+            if bad and tl["kind"] == 0:
+                info = info or int(pn["c0"]) + bad
     return L, info
+
+
+def _panel_factor_scalar(L, base, ld, a11, nc, nr, first, xsize):
+    """k_panel_factor_regs: right-looking over single columns."""
+    l11 = np.zeros_like(a11)
+    bad = 0
+    for j in range(nc):
+        d = a11[j, j]
+        if not d > 0:
+            bad = bad or j + 1
+            d = 1.0
+        l11[j, j] = np.sqrt(d) if a11[j, j] > 0 else 1.0
+        l11[j + 1:, j] = a11[j + 1:, j] / l11[j, j]
+        for c in range(j + 1, nc):
+            a11[c:, c] -= l11[c:, j] * l11[c, j]
+    if first == nc:
+        keep = np.tril(np.ones((nc, nc), bool))
+        didx = base + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
+        L[didx[keep]] = l11[keep]
+    if nr > first:
+        idx = base + np.arange(nc)[None, :] * ld + np.arange(first, nr)[:, None]
+        _chk(int(idx.min()), int(idx.max()) + 1, xsize, "factor tile")
+        L[idx] = np.linalg.solve(l11, L[idx].T).T
+    return bad
+
+
+KWLD = 2 * 64 + 1  # column stride of the LDS working matrix of k_panel_factor_blk
+
+
+def panel_factor_blk(L, base, ld, a11, nc, nr, first, xsize):
+    """k_panel_factor_blk with its LDS arrays W (64 columns of KWLD doubles) and
+    lc (3 x 16 x 16), every LDS and global index bounds-checked: the per-wave
+    column loads, per block column the sweep of waves 0-2 (lane = row; wave 0
+    the diagonal block's rows, waves 1-2 the 16 pivot rows in lanes 0..15 and
+    the tile rows after them), the rank-16 updates of the later block columns in
+    16x16 MFMA output blocks, and the per-wave column stores."""
+    W = np.full(64 * KWLD, np.nan)
+    lc = np.full((3, 16, 16), np.nan)
+    nb = (nc + 15) >> 4
+    assert 2 <= nb <= 4
+    nx = max(nr - first, 0)
+    nxb = (nx + 15) >> 4
+    assert nx <= 64
+    lanes = np.arange(64)
+
+    def w(c, r):
+        c, r = np.asarray(c), np.asarray(r)
+        if c.size and r.size:
+            assert c.min() >= 0 and c.max() < 64 and r.min() >= 0 and r.max() < 128
+        return c * KWLD + r
+
+    for wave in range(4):
+        c0 = 16 * wave
+        for c in range(16):
+            col = c0 + c
+            iny = (lanes < nc) & (col < nc) & (col <= lanes)
+            inx = (lanes < nx) & (col < nc)
+            gy = np.where(iny, col * nc + lanes, 0)
+            assert gy.min() >= 0 and gy.max() < nc * nc
+            gx = base + np.where(inx, col * ld + first + lanes, 0)
+            _chk(int(gx.min()), int(gx.max()) + 1, xsize, "factor load")
+            y = np.where(iny, a11[np.minimum(lanes, nc - 1), min(col, nc - 1)],
+                         np.where(((lanes >= nc) | (col >= nc)) & (col == lanes), 1.0, 0.0))
+            x = np.where(inx, L[gx], 0.0)
+            if c0 < 16 * nb:
+                W[w(col, lanes)] = y
+                W[w(col, 64 + lanes)] = x
+    bad = 0
+    for J in range(nb):
+        jb = 16 * J
+        sweeps = []
+        for wave in range(3):
+            if wave == 0:
+                row = lanes.copy()
+                pbase = jb
+                store = (lanes >= jb) & (lanes < 16 * nb)
+            else:
+                dup = (lanes < 16) | ((wave == 2) & (lanes >= 32))
+                row = np.where(dup, jb + (lanes & 15), 64 + 48 * (wave - 1) + (lanes - 16))
+                pbase = 0
+                store = (lanes >= 16) & (row - 64 < nx) & ((wave == 1) | (lanes < 32))
+            # every wave reads its rows before any wave writes (a barrier)
+            sweeps.append((row, pbase, store, np.stack([W[w(jb + c, row)] for c in range(16)], axis=1)))
+        w(jb + 15, jb + (lanes & 15))  # wave 3 reads one row too
+        for wave, (row, pbase, store, X) in enumerate(sweeps):
+            assert 0 <= pbase and pbase + 15 < 64
+            d = X[pbase, 0]
+            for j in range(16):
+                if d > 0:
+                    X[:, j] *= 1.0 / np.sqrt(d)
+                else:
+                    bad = bad or jb + j + 1
+                    X[pbase + j, j] = 1.0
+                if j < 15:
+                    d = X[pbase + j + 1, j + 1] - X[pbase + j + 1, j] * X[pbase + j + 1, j]
+                    sel = (lanes >= pbase) & (lanes < pbase + 16)
+                    idx = lanes[sel] - pbase
+                    assert idx.min() >= 0 and idx.max() < 16
+                    lc[wave, j, idx] = X[sel, j]
+                    for c in range(j + 1, 16):
+                        X[:, c] -= X[:, j] * lc[wave, j, c]
+            for c in range(16):
+                W[w(jb + c, row[store])] = X[store, c]
+        for C in range(J + 1, nb):
+            for R in list(range(C, nb)) + list(range(4, 4 + nxb)):
+                for r in range(4):
+                    i = 16 * R + (lanes >> 4) + 4 * r
+                    jcol = 16 * C + (lanes & 15)
+                    acc = W[w(jcol, i)].copy()
+                    for u in range(4):
+                        k = jb + (lanes >> 4) + 4 * u
+                        # lane l supplies A[row 16R + l%16][k] and B[row 16C + l%16][k]
+                        assert k.max() < jb + 16
+                    A = np.stack([W[w(jb + kk, i)] for kk in range(16)], axis=1)
+                    B = np.stack([W[w(jb + kk, jcol)] for kk in range(16)], axis=1)
+                    W[w(jcol, i)] = acc - (A * B).sum(axis=1)
+    for wave in range(4):
+        c0 = 16 * wave
+        for c in range(16):
+            col = c0 + c
+            if col >= nc:
+                continue
+            if first == nc:
+                sd = (lanes < nc) & (col <= lanes)
+                gi = base + col * ld + lanes[sd]
+                if len(gi):
+                    _chk(int(gi.min()), int(gi.max()) + 1, xsize, "factor store")
+                L[gi] = W[w(col, lanes[sd])]
+            sx = lanes < nx
+            gi = base + col * ld + first + lanes[sx]
+            if len(gi):
+                _chk(int(gi.min()), int(gi.max()) + 1, xsize, "factor store")
+            L[gi] = W[w(col, 64 + lanes[sx])]
+    return bad
 
 
 def solve(plan, Ls, L, b):
