@@ -293,9 +293,9 @@ PANEL_DT = np.dtype([("vbase", np.int64), ("voff", np.int64), ("ld", np.int32),
                      ("ncol", np.int32), ("nrow", np.int32), ("c0", np.int32),
                      ("rows_off", np.int32), ("sn", np.int32), ("sc0", np.int32),
                      ("sw", np.int32)])
-TILE_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
+TILE_DT = np.dtype([("vbase", np.int64), ("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("task_begin", np.int32), ("task_end", np.int32), ("kind", np.int32),
-                    ("slot", np.int32), ("pad", np.int32)])
+                    ("slot", np.int32), ("ld", np.int32), ("ncol", np.int32), ("c0", np.int32)])
 TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rows_off", np.int32), ("ra", np.int32), ("rb", np.int32),
                     ("ca", np.int32), ("cb", np.int32), ("kind", np.int32),

@@ -199,18 +199,17 @@ __device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
 // for every tile.
 template <int MODE>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
-k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
-              const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
-              double* __restrict__ dscratch, const float* __restrict__ Lf) {
+k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
+              const Task* __restrict__ tasks, int tile0, double* __restrict__ dscratch,
+              const float* __restrict__ Lf) {
   __shared__ double red[2][swc::PW * kLdsLd];
   const Tile tl = tiles[tile0 + blockIdx.x];
-  const Panel pn = panels[tl.panel];
   // the wave index is uniform over the wave: say so, so that the per-chunk
   // buffer resources stay in scalar registers
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nr = tl.nr, nc = pn.ncol;
+  const int nr = tl.nr, nc = tl.ncol, ld = tl.ld;
   const bool diag = tl.r0 == 0;
-  double* gtile = L + pn.vbase + tl.r0;
+  double* gtile = L + tl.vbase + tl.r0;
   d4 acc[4][4];
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi)
@@ -231,8 +230,17 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
                           wave, lane, acc);
     }
   }
+  // the tile's current values, loaded before the reduction so that their
+  // latency overlaps it: thread t owns elements t, t + 256, ... of the
+  // column-major tile
+  double cur[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
+    cur[q] = (i < nr && j < nc) ? gtile[(int64_t)j * ld + i] : 0.0;
+  }
   // partial sums (w0 + w1) and (w2 + w3), then their sum, element-wise by all
-  // threads: thread t owns elements t, t + 256, ... of the column-major tile
+  // threads
   double s01[16];
   if (wave < 2) acc_store(acc, red[wave], lane);
   __syncthreads();
@@ -255,8 +263,8 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
     if (i < nr && j < nc) {
       const double s23 = red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i];
       if (!(diag && i < j)) {
-        const double v = gtile[(int64_t)j * pn.ld + i] - (s01[q] + s23);
-        gtile[(int64_t)j * pn.ld + i] = v;
+        const double v = cur[q] - (s01[q] + s23);
+        gtile[(int64_t)j * ld + i] = v;
         if (copy && i < nc) dscratch[tl.slot + j * nc + i] = v;
       } else if (copy) {
         dscratch[tl.slot + j * nc + i] = 0.0;
@@ -271,15 +279,14 @@ k_tile_update(double* __restrict__ L, const Panel* __restrict__ panels,
 // rules as k_tile_update.
 template <int MODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-k_tile_narrow(double* __restrict__ L, const Panel* __restrict__ panels,
-              const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
-              double* __restrict__ dscratch, const float* __restrict__ Lf) {
+k_tile_narrow(double* __restrict__ L, const Tile* __restrict__ tiles,
+              const Task* __restrict__ tasks, int tile0, double* __restrict__ dscratch,
+              const float* __restrict__ Lf) {
   const Tile tl = tiles[tile0 + blockIdx.x];
-  const Panel pn = panels[tl.panel];
   const int lane = threadIdx.x;
-  const int nr = tl.nr, nc = pn.ncol;
+  const int nr = tl.nr, nc = tl.ncol, ld = tl.ld;
   const bool diag = tl.r0 == 0;
-  double* gtile = L + pn.vbase + tl.r0;
+  double* gtile = L + tl.vbase + tl.r0;
   d4 acc[4][1];
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi) acc[bi][0] = d4{0.0, 0.0, 0.0, 0.0};
@@ -307,8 +314,8 @@ k_tile_narrow(double* __restrict__ L, const Panel* __restrict__ panels,
       const int i = 16 * bi + (lane >> 4) + 4 * r;
       if (i < nr && j < nc) {
         if (!(diag && i < j)) {
-          const double v = gtile[(int64_t)j * pn.ld + i] - acc[bi][0][r];
-          gtile[(int64_t)j * pn.ld + i] = v;
+          const double v = gtile[(int64_t)j * ld + i] - acc[bi][0][r];
+          gtile[(int64_t)j * ld + i] = v;
           if (copy && i < nc) dscratch[tl.slot + j * nc + i] = v;
         } else if (copy) {
           dscratch[tl.slot + j * nc + i] = 0.0;
@@ -340,14 +347,12 @@ __device__ inline double bcast(double v, int src) {
 // it, so they are left unmasked.
 template <int NC>
 __global__ void __launch_bounds__(64)
-k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
-               const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
-               const double* __restrict__ dscratch, int* __restrict__ info_out,
-               float* __restrict__ Lf) {
-  const Tile tl = tiles[list[list0 + blockIdx.x]];
-  const Panel pn = panels[tl.panel];
+k_panel_factor_regs(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
+                    const double* __restrict__ dscratch, int* __restrict__ info_out,
+                    float* __restrict__ Lf) {
+  const Tile tl = ftiles[list0 + blockIdx.x];
   const int lane = threadIdx.x;
-  const int nr = tl.nr, nc = pn.ncol;
+  const int nr = tl.nr, nc = tl.ncol;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
   double y[NC];                              // row `lane` of the diagonal block
 #pragma unroll
@@ -369,15 +374,15 @@ k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
         if (c < nc) y[c] -= y[j] * bcast(y[j], c);
     }
   }
-  double* gtile = L + pn.vbase + tl.r0;
-  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
+  double* gtile = L + tl.vbase + tl.r0;
+  float* ftile = Lf ? Lf + tl.vbase + tl.r0 : nullptr;
   if (tl.kind == 0) {
-    if (badcol && lane == 0) atomicMin(info_out, pn.c0 + badcol);
+    if (badcol && lane == 0) atomicMin(info_out, tl.c0 + badcol);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       if (lane < nc && c <= lane) {
-        gtile[(int64_t)c * pn.ld + lane] = y[c];
-        if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)y[c];
+        gtile[(int64_t)c * tl.ld + lane] = y[c];
+        if (ftile) ftile[(int64_t)c * tl.ld + lane] = (float)y[c];
       }
   }
   // rows of this tile below the diagonal block: tile rows [first, nr)
@@ -385,7 +390,7 @@ k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
   const bool mine = lane >= first && lane < nr;
   double x[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * pn.ld + lane] : 0.0;
+  for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? gtile[(int64_t)c * tl.ld + lane] : 0.0;
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     if (j < nc) {
@@ -398,8 +403,8 @@ k_panel_factor_regs(double* __restrict__ L, const Panel* __restrict__ panels,
 #pragma unroll
   for (int c = 0; c < NC; ++c)
     if (mine && c < nc) {
-      gtile[(int64_t)c * pn.ld + lane] = x[c];
-      if (ftile) ftile[(int64_t)c * pn.ld + lane] = (float)x[c];
+      gtile[(int64_t)c * tl.ld + lane] = x[c];
+      if (ftile) ftile[(int64_t)c * tl.ld + lane] = (float)x[c];
     }
 }
 
@@ -450,22 +455,20 @@ __device__ inline d4 lds_block_update(const double* __restrict__ W, d4 acc, int 
 // tile of a panel computes the same L11, so no tile waits for another; the
 // diagonal tile stores it and reports info.
 __global__ void __launch_bounds__(256)
-k_panel_factor_blk(double* __restrict__ L, const Panel* __restrict__ panels,
-                   const Tile* __restrict__ tiles, const int* __restrict__ list, int list0,
+k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
                    const double* __restrict__ dscratch, int* __restrict__ info_out,
                    float* __restrict__ Lf) {
   __shared__ double W[swc::PW * kWld];
   __shared__ double lc[3][16][16];  // per sweeping wave: column j's multipliers
-  const Tile tl = tiles[list[list0 + blockIdx.x]];
-  const Panel pn = panels[tl.panel];
+  const Tile tl = ftiles[list0 + blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nc = pn.ncol;
+  const int nc = tl.ncol;
   const int nb = (nc + 15) >> 4;              // 16-column blocks (2..4)
   const int first = tl.kind == 0 ? nc : 0;    // tile rows [first, nr) are solved ...
   const int nx = tl.nr > first ? tl.nr - first : 0;  // ... into W rows 64 .. 64 + nx
   const int nxb = (nx + 15) >> 4;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
-  double* gtile = L + pn.vbase + tl.r0;
+  double* gtile = L + tl.vbase + tl.r0;
   {
     // wave w moves columns 16w..16w+15 of diagonal-block row `lane` and of tile
     // row `lane`; every lane loads from an address inside both buffers and then
@@ -477,7 +480,7 @@ k_panel_factor_blk(double* __restrict__ L, const Panel* __restrict__ panels,
       const int col = c0 + c;
       const bool iny = lane < nc && col < nc && col <= lane, inx = lane < nx && col < nc;
       const double vy = gdiag[iny ? col * nc + lane : 0];
-      const double vx = gtile[inx ? (int64_t)col * pn.ld + first + lane : 0];
+      const double vx = gtile[inx ? (int64_t)col * tl.ld + first + lane : 0];
       y[c] = iny ? vy : ((lane >= nc || col >= nc) && col == lane ? 1.0 : 0.0);
       x[c] = inx ? vx : 0.0;
     }
@@ -556,8 +559,8 @@ k_panel_factor_blk(double* __restrict__ L, const Panel* __restrict__ panels,
       __syncthreads();
     }
   }
-  if (tl.kind == 0 && badcol && badcol <= nc && tid == 0) atomicMin(info_out, pn.c0 + badcol);
-  float* ftile = Lf ? Lf + pn.vbase + tl.r0 : nullptr;
+  if (tl.kind == 0 && badcol && badcol <= nc && tid == 0) atomicMin(info_out, tl.c0 + badcol);
+  float* ftile = Lf ? Lf + tl.vbase + tl.r0 : nullptr;
   const int c0 = 16 * wave;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
@@ -565,13 +568,13 @@ k_panel_factor_blk(double* __restrict__ L, const Panel* __restrict__ panels,
     if (col < nc) {
       if (tl.kind == 0 && lane < nc && col <= lane) {
         const double v = W[col * kWld + lane];
-        gtile[(int64_t)col * pn.ld + lane] = v;
-        if (ftile) ftile[(int64_t)col * pn.ld + lane] = (float)v;
+        gtile[(int64_t)col * tl.ld + lane] = v;
+        if (ftile) ftile[(int64_t)col * tl.ld + lane] = (float)v;
       }
       if (lane < nx) {
         const double v = W[col * kWld + 64 + lane];
-        gtile[(int64_t)col * pn.ld + first + lane] = v;
-        if (ftile) ftile[(int64_t)col * pn.ld + first + lane] = (float)v;
+        gtile[(int64_t)col * tl.ld + first + lane] = v;
+        if (ftile) ftile[(int64_t)col * tl.ld + first + lane] = (float)v;
       }
     }
   }
@@ -1577,7 +1580,7 @@ struct swc_handle {
   int* dSpOrder = nullptr;    // solve panels by level
   double* dY = nullptr;   // forward solutions, n * work_cols
   int* dOrder = nullptr;
-  int* dTrsm = nullptr;
+  Tile* dFtiles = nullptr;  // factor-kernel tiles (copies), by level and width class
   double* dDiag = nullptr;  // per-level copies of the diagonal blocks being factored
   int64_t* dAmap = nullptr;
   int* dPerm = nullptr;
@@ -1651,14 +1654,14 @@ struct swc_handle {
   void release() {
     drop_graphs();
     (void)hipFree(dL); (void)hipFree(dLf); (void)hipFree(dLs); (void)hipFree(dPanels); (void)hipFree(dTiles); (void)hipFree(dTasks);
-    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dTrsm); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
+    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dFtiles); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
     (void)hipFree(dVals); (void)hipFree(dWork); (void)hipFree(dInfo);
     (void)hipFree(dRp); (void)hipFree(dCi); (void)hipFree(dVmap); (void)hipFree(dAf); (void)hipFree(dNorm); (void)hipFree(dScal); (void)hipFree(dPart); (void)hipFree(dNormPart);
     (void)hipFree(dRef);
     (void)hipFree(dZ); (void)hipFree(dArena); (void)hipFree(dNodes); (void)hipFree(dCol2sup); (void)hipFree(dSpx);
     (void)hipFree(dSsuper); (void)hipFree(dSpi); (void)hipFree(dDiagPos); (void)hipFree(dQi); (void)hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dTrsm = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dFtiles = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dNorm = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     lf_cap = 0;
@@ -1770,31 +1773,27 @@ void enqueue_factor(swc_handle* h, bool profile) {
     if (ntn > 0) {
       auto kern = h->precision == 0 ? k_tile_narrow<0> : (h->precision == 1 ? k_tile_narrow<1>
                                                                              : k_tile_narrow<2>);
-      hipLaunchKernelGGL(kern, dim3(ntn), dim3(64), 0, st, h->dL, (const Panel*)h->dPanels,
-                         (const Tile*)h->dTiles, (const Task*)h->dTasks, t0, h->dDiag,
-                         (const float*)h->dLf);
+      hipLaunchKernelGGL(kern, dim3(ntn), dim3(64), 0, st, h->dL, (const Tile*)h->dTiles,
+                         (const Task*)h->dTasks, t0, h->dDiag, (const float*)h->dLf);
     }
     if (nt > ntn) {
       auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
                                                                              : k_tile_update<2>);
-      hipLaunchKernelGGL(kern, dim3(nt - ntn), dim3(kThreads), 0, st, h->dL,
-                         (const Panel*)h->dPanels, (const Tile*)h->dTiles, (const Task*)h->dTasks,
-                         t0 + ntn, h->dDiag, (const float*)h->dLf);
+      hipLaunchKernelGGL(kern, dim3(nt - ntn), dim3(kThreads), 0, st, h->dL, (const Tile*)h->dTiles,
+                         (const Task*)h->dTasks, t0 + ntn, h->dDiag, (const float*)h->dLf);
     }
     if (pe) (void)hipEventRecord(pe[1], st);
     if (ns > 0) {
       // the level's panels by width class: <= 16, <= 32, <= 64 columns
       const int* cls = &P.lvl_trsm_cls[(size_t)4 * l];
       float* lf = h->fp32_mask ? h->dLf : nullptr;
-      const Panel* pp = (const Panel*)h->dPanels;
-      const Tile* tp = (const Tile*)h->dTiles;
-      const int* lp = (const int*)h->dTrsm;
+      const Tile* ft = (const Tile*)h->dFtiles;
       if (cls[1] > cls[0])
-        hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, pp, tp,
-                           lp, cls[0], (const double*)h->dDiag, h->dInfo, lf);
+        hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, ft,
+                           cls[0], (const double*)h->dDiag, h->dInfo, lf);
       if (cls[3] > cls[1])
-        hipLaunchKernelGGL(k_panel_factor_blk, dim3(cls[3] - cls[1]), dim3(256), 0, st, h->dL, pp,
-                           tp, lp, cls[1], (const double*)h->dDiag, h->dInfo, lf);
+        hipLaunchKernelGGL(k_panel_factor_blk, dim3(cls[3] - cls[1]), dim3(256), 0, st, h->dL, ft,
+                           cls[1], (const double*)h->dDiag, h->dInfo, lf);
     }
     if (pe) (void)hipEventRecord(pe[2], st);
   }
@@ -2343,7 +2342,7 @@ int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const
   if (!rc) rc = upload(&h->dSpanels, P.spanels.data(), P.spanels.size(), st);
   if (!rc) rc = upload(&h->dSpOrder, P.sp_order.data(), P.sp_order.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
-  if (!rc) rc = upload(&h->dTrsm, P.trsm_tiles.data(), P.trsm_tiles.size(), st);
+  if (!rc) rc = upload(&h->dFtiles, P.ftiles.data(), P.ftiles.size(), st);
   if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
   if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
   if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);

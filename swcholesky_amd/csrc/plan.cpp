@@ -163,12 +163,15 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   auto add_tile = [&](int q, int r0, int kind, int level, bool trsm) {
     const Panel& pn = P.panels[q];
     PTile pt;
+    pt.t.vbase = pn.vbase;
     pt.t.panel = q;
     pt.t.r0 = r0;
     pt.t.nr = std::min(TM, pn.nrow - r0);
     pt.t.kind = kind;
     pt.t.slot = -1;
-    pt.t.pad = 0;
+    pt.t.ld = pn.ld;
+    pt.t.ncol = pn.ncol;
+    pt.t.c0 = pn.c0;
     pt.t.task_begin = pt.t.task_end = 0;
     pt.level = level;
     pt.trsm = trsm;
@@ -477,6 +480,8 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     P.trsm_tiles.insert(P.trsm_tiles.end(), v.begin(), v.end());
     P.lvl_trsm[l + 1] = (int)P.trsm_tiles.size();
   }
+  P.ftiles.reserve(P.trsm_tiles.size());
+  for (int ti : P.trsm_tiles) P.ftiles.push_back(P.tiles[ti]);
   for (const Panel& pn : P.panels) {
     const double w = pn.ncol, m = pn.nrow - pn.ncol;
     P.factor_flops += w * w * w / 3.0 + m * w * w;

@@ -38,6 +38,7 @@ struct Panel {
 };
 
 struct Tile {
+  int64_t vbase;   // the panel's value offset of (row 0, column 0)
   int panel;
   int r0;          // first panel row of the tile
   int nr;          // rows in the tile (<= TM)
@@ -46,7 +47,9 @@ struct Tile {
   int kind;        // 0: the panel's diagonal tile, 1: below it
   int slot;        // factor level: offset of the panel's diagonal block copy in the
                    // per-level scratch (updated, not yet factored), -1 otherwise
-  int pad;
+  int ld;          // the panel's leading dimension, columns and first column (copied
+  int ncol;        // from the panel, so that a kernel reads one descriptor)
+  int c0;
 };
 
 // One source contribution to one tile: rows [ra,rb) x columns [ca,cb) of the
@@ -77,7 +80,7 @@ struct SolveBlock {
 };
 
 static_assert(sizeof(Panel) == 48, "Panel layout is part of the plan export");
-static_assert(sizeof(Tile) == 32, "Tile layout is part of the plan export");
+static_assert(sizeof(Tile) == 48, "Tile layout is part of the plan export");
 static_assert(sizeof(Task) == 56, "Task layout is part of the plan export");
 static_assert(sizeof(SolveBlock) == 16, "SolveBlock layout is part of the plan export");
 
@@ -90,6 +93,7 @@ struct Plan {
   std::vector<int> lvl_tiles_narrow;  // nlevel: one-wave tiles (panels <= 16 wide, short
                                       // total inner dimension), first in their level
   std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)
+  std::vector<Tile> ftiles;         // copies of those tiles in the same order
   std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles
   std::vector<int> lvl_trsm_cls;    // 4 per level: starts of the panel-width classes
                                     // (<= 16, <= 32, <= 64 columns) in trsm_tiles, and the end
