@@ -177,7 +177,7 @@ __device__ inline void task_update(const T* __restrict__ src, int ld, int k, int
   }
 }
 
-// acc (f64 MFMA layout) <-> a 64x64 LDS tile (column j at buf[j * kLdsLd])
+// acc (f64 MFMA layout) -> a 64x64 LDS tile (column j at buf[j * kLdsLd])
 __device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi)
@@ -188,10 +188,21 @@ __device__ inline void acc_store(const d4 (&acc)[4][4], double* buf, int lane) {
         buf[(16 * bj + (lane & 15)) * kLdsLd + 16 * bi + (lane >> 4) + 4 * r] = acc[bi][bj][r];
 }
 
+// the LDS tile += acc, element by element
+__device__ inline void acc_add(const d4 (&acc)[4][4], double* buf, int lane) {
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        buf[(16 * bj + (lane & 15)) * kLdsLd + 16 * bi + (lane >> 4) + 4 * r] += acc[bi][bj][r];
+}
+
 // Tile kernel: applies all source updates of one tile, in task order.  Every
 // wave accumulates the products of a fixed share of each task's inner
 // dimension over the whole 64x64 tile in MFMA registers; the four partial sums
-// are combined in a fixed order, (w0 + w1) + (w2 + w3), through LDS and
+// are combined in a fixed order, (w0 + w2) + (w1 + w3), through LDS and
 // subtracted from the tile, so results do not depend on scheduling.  The
 // factorization of panels completed by this level follows in the panel factor kernels.
 // MODE 0: FP64 throughout; 1: descendant (kind 0) tiles on the FP32 copy,
@@ -239,18 +250,11 @@ k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
     const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
     cur[q] = (i < nr && j < nc) ? gtile[(int64_t)j * ld + i] : 0.0;
   }
-  // partial sums (w0 + w1) and (w2 + w3), then their sum, element-wise by all
-  // threads
-  double s01[16];
+  // partial sums (w0 + w2) and (w1 + w3) in LDS, added element-wise by all
+  // threads below
   if (wave < 2) acc_store(acc, red[wave], lane);
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
-    s01[q] = red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i];
-  }
-  __syncthreads();
-  if (wave >= 2) acc_store(acc, red[wave - 2], lane);
+  if (wave >= 2) acc_add(acc, red[wave - 2], lane);
   __syncthreads();
   // write back tile - sum; rows 0..nc-1 of a tile at row offset 0 are the
   // panel's diagonal block, whose strict upper part stays untouched (zero).
@@ -261,9 +265,8 @@ k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
   for (int q = 0; q < 16; ++q) {
     const int e = tid + q * kThreads, i = e & 63, j = e >> 6;
     if (i < nr && j < nc) {
-      const double s23 = red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i];
       if (!(diag && i < j)) {
-        const double v = cur[q] - (s01[q] + s23);
+        const double v = cur[q] - (red[0][j * kLdsLd + i] + red[1][j * kLdsLd + i]);
         gtile[(int64_t)j * ld + i] = v;
         if (copy && i < nc) dscratch[tl.slot + j * nc + i] = v;
       } else if (copy) {
