@@ -243,6 +243,8 @@ def main():
         "config": {"workload": f"lap3d_var_{args.side}: 7-point Laplacian {args.side}^3, "
                                "weights log-uniform [1e-2,1e2] seed 1, shift 1e-6, ND ordering",
                    "n": prob.n, "nnz_tril": prob.nnz, "nsuper": b.nsuper, "flops_F": b.flops,
+                   "plan_params": dict(zip(("rlb", "spw", "narrow_min_tiles", "narrow_max_k"),
+                                           f.params or ())),
                    "parallelism": "single GPU"},
         "time_to_1e-12_ms": {k: v["ms"] for k, v in paths.items()},
         "mixed_speedup_vs_fp64": paths["fp64"]["ms"] / min(paths["mixed"]["ms"], paths["mixed_all"]["ms"]),

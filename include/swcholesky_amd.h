@@ -120,6 +120,17 @@ int swc_upload_symbolic(swc_handle* h, const swc_symbolic* sym);
  * NULL clears it (solves then work in the factor's ordering). */
 int swc_set_perm(swc_handle* h, const int* perm);
 
+/* Shape parameters of the plan built by the next structure upload (swc_upload,
+ * swc_upload_symbolic), as {rlb, spw, narrow_min_tiles, narrow_max_k} (count <= 4,
+ * leading entries; NULL restores the defaults {8, 128, 1024, 256}): rlb panels per
+ * block of a wide supernode's right-looking steps, spw = 64 or 128 columns per
+ * solve panel, and the one-wave tile kernel for levels with at least
+ * narrow_min_tiles tiles of panels <= 16 columns whose inner dimensions sum to
+ * <= narrow_max_k.  No reference counterpart (the reference's Sunway BLAS fixes
+ * its blocking, BLAS-sw/dpotrf_cluster_master.c:11-63); the values come from the
+ * autotuner's table (swcholesky_amd/tuning.json). */
+int swc_set_plan_params(swc_handle* h, const int* params, int count);
+
 /* Precision of the next swc_factor calls.  0 (default): FP64 throughout.
  * 1: descendant SYRK/GEMM updates on FP32 MFMA with FP32 accumulation, each
  *    update subtracted into the FP64 target; POTRF, TRSM and the updates inside
@@ -211,6 +222,10 @@ typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out);
+/* The same with plan shape parameters (as swc_set_plan_params). */
+int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
+                          const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
+                          const int* params, int nparams, swc_plan** out);
 void swc_plan_free(swc_plan* p);
 int swc_plan_sizes(const swc_plan* p, int64_t* sizes, double* flops);
 int swc_plan_export(const swc_plan* p, int which, void* dst);

@@ -148,7 +148,10 @@ class Factor:
     solves A x = b in the original ordering.
     """
 
-    def __init__(self, bcsc: BCSC):
+    def __init__(self, bcsc: BCSC, params="auto"):
+        """params: "auto" (the autotuner's table entry for this structure's class,
+        swcholesky_amd/tuning.py), None (the library defaults) or a sequence
+        {rlb, spw, narrow_min_tiles, narrow_max_k} (swc_set_plan_params)."""
         self.bcsc = bcsc
         self.n = bcsc.n
         lib = _lib.lib()
@@ -156,6 +159,13 @@ class Factor:
         check(lib.swc_create(ctypes.byref(h)))
         self._destroy = lib.swc_destroy  # see BCSC
         self._h = h
+        if isinstance(params, str) and params == "auto":
+            from .tuning import params_for
+            params = params_for(bcsc)
+        self.params = None if params is None else tuple(int(v) for v in params)
+        if self.params is not None:
+            pv = np.asarray(self.params, dtype=np.int32)
+            check(lib.swc_set_plan_params(self._h, ptr(pv), len(pv)))
         check(lib.swc_upload_symbolic(self._h, bcsc._h))
         self.info = None
 
@@ -304,10 +314,11 @@ SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("writer", np.int32)])
 
 
-def build_plan(bcsc: BCSC, derive_sets=False):
+def build_plan(bcsc: BCSC, derive_sets=False, params=None):
     """The device work plan for a structure, as numpy record arrays (host only).
     derive_sets: ignore the analysis's descendant sets and derive them from the
-    row lists (what blockedLsolve does)."""
+    row lists and build the solve part only (what blockedLsolve does); params:
+    plan shape parameters as swc_set_plan_params takes them."""
     lib = _lib.lib()
     sizes = np.zeros(4, dtype=np.int64)
     check(lib.swc_plan_struct_sizes(ptr(sizes)))
@@ -319,8 +330,10 @@ def build_plan(bcsc: BCSC, derive_sets=False):
     p = np.ascontiguousarray(bcsc.p.astype(np.uint64))
     iptr = np.ascontiguousarray(bcsc.i_ptr.astype(np.uint64))
     pp, ps = (None, None) if derive_sets else (bcsc.prunePtr, bcsc.pruneSet)
-    check(lib.swc_plan_build(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s), ptr(iptr),
-                             ptr(pp), ptr(ps), ctypes.byref(h)))
+    pv = None if params is None else np.asarray(params, dtype=np.int32)
+    check(lib.swc_plan_build_params(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s),
+                                    ptr(iptr), ptr(pp), ptr(ps), ptr(pv),
+                                    0 if pv is None else len(pv), ctypes.byref(h)))
     try:
         sz = np.zeros(12, dtype=np.int64)
         fl = (ctypes.c_double * 2)()

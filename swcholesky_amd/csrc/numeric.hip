@@ -1558,6 +1558,7 @@ struct swc_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   Plan plan;
+  swc::PlanParams params;  // shape parameters of the next structure upload
   int n = 0, nsuper = 0;
   int64_t nvals = 0;  // number of values swc_factor expects
   std::vector<int> super, Ls;
@@ -2314,7 +2315,7 @@ int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const
   h->pruneset.assign(pruneSet, pruneSet + prunePtr[supNo]);
   try {
     swc::build_plan(n, supNo, h->super.data(), h->pi.data(), h->px.data(), h->Ls.data(),
-                    prunePtr, pruneSet, h->plan, factor);
+                    prunePtr, pruneSet, h->plan, factor, h->params);
   } catch (const std::bad_alloc&) {
     return swc_set_error(SWC_ENOMEM, "upload: out of host memory");
   } catch (const std::exception& e) {
@@ -2643,6 +2644,15 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
     const double den = h->normA * xn[c] + bn[c];
     eta[c] = den > 0 ? rn[c] / den : 0.0;
   }
+  return SWC_OK;
+}
+
+int swc_set_plan_params(swc_handle* h, const int* params, int count) {
+  if (int rc = check_handle(h)) return rc;
+  swc::PlanParams p;
+  if (params && !swc::plan_params_from(params, count, p))
+    return swc_set_error(SWC_EINVAL, "swc_set_plan_params: parameter out of range");
+  h->params = p;  // NULL: the defaults
   return SWC_OK;
 }
 

@@ -56,8 +56,23 @@ void descendant_sets(int n, int nsuper, const int* super, const int64_t* pi, con
   }
 }
 
+bool plan_params_from(const int* v, int count, PlanParams& out) {
+  PlanParams p;
+  if (count > 0) p.rlb = v[0];
+  if (count > 1) p.spw = v[1];
+  if (count > 2) p.narrow_min_tiles = v[2];
+  if (count > 3) p.narrow_max_k = v[3];
+  if (count < 0 || count > kPlanParams || p.rlb < 1 || p.rlb > 64 ||
+      (p.spw != 64 && p.spw != SPW) || p.narrow_min_tiles < 1 || p.narrow_max_k < 0)
+    return false;
+  out = p;
+  return true;
+}
+
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
-                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P, bool factor) {
+                const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P, bool factor,
+                const PlanParams& params) {
+  const int RLB = params.rlb, SPWP = params.spw;
   P = Plan();
   P.n = n;
   P.nsuper = nsuper;
@@ -279,11 +294,11 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       const int W = super[s + 1] - super[s];
       const int64_t R = pi[s + 1] - pi[s];
       int lvl = first;
-      for (int c = 0; c < W; c += SPW, ++lvl) {
+      for (int c = 0; c < W; c += SPWP, ++lvl) {
         Panel sp;
         sp.vbase = px[s] + (int64_t)c * R + c;
         sp.ld = (int)R;
-        sp.ncol = std::min(SPW, W - c);
+        sp.ncol = std::min(SPWP, W - c);
         sp.nrow = (int)R - c;
         sp.c0 = super[s] + c;
         sp.rows_off = (int)(pi[s] + c);
@@ -365,21 +380,21 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   std::vector<int> order(ptiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   // Tiles of panels at most 16 wide whose tasks have a short total inner
-  // dimension (at most 256) come first in their level, where the level has at
-  // least 1024 of them: they run one wave each in their own launch.  The others
-  // keep the four-wave split of the inner dimension.
+  // dimension (at most narrow_max_k) come first in their level, where the level
+  // has at least narrow_min_tiles of them: they run one wave each in their own
+  // launch.  The others keep the four-wave split of the inner dimension.
   std::vector<char> is_narrow(ptiles.size(), 0);
   for (size_t i = 0; i < ptiles.size(); ++i) {
     int64_t sumk = 0;
     for (const Task& t : ptasks[i]) sumk += t.k;
-    is_narrow[i] = P.panels[ptiles[i].t.panel].ncol <= 16 && sumk <= 256;
+    is_narrow[i] = P.panels[ptiles[i].t.panel].ncol <= 16 && sumk <= params.narrow_max_k;
   }
   {
     // a separate launch only pays where a level has many of them
     std::vector<int> cnt(P.nlevel, 0);
     for (size_t i = 0; i < ptiles.size(); ++i) cnt[ptiles[i].level] += is_narrow[i];
     for (size_t i = 0; i < ptiles.size(); ++i)
-      if (cnt[ptiles[i].level] < 1024) is_narrow[i] = 0;
+      if (cnt[ptiles[i].level] < params.narrow_min_tiles) is_narrow[i] = 0;
   }
   auto narrow = [&](int i) { return is_narrow[i] != 0; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -503,6 +518,16 @@ extern "C" {
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out) {
+  return swc_plan_build_params(n, blockSet, supNo, lC, Ls, Li_ptr, prunePtr, pruneSet, nullptr, 0,
+                               out);
+}
+
+int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
+                          const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
+                          const int* params, int nparams, swc_plan** out) {
+  swc::PlanParams prm;
+  if (params && !swc::plan_params_from(params, nparams, prm))
+    return swc_set_error(SWC_EINVAL, "swc_plan_build: plan parameter out of range");
   // pruneSet may be null when it is empty (no supernode has descendants); with
   // prunePtr null both are derived from the row lists and only the solve part of
   // the plan is built (what blockedLsolve does)
@@ -522,7 +547,8 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
       pruneSet = ps.data();
     }
     auto* h = new swc_plan;
-    swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P, factor);
+    swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P, factor,
+                    prm);
     *out = h;
     return SWC_OK;
   } catch (const std::exception& e) {

@@ -9,7 +9,7 @@
 // tile then factors the panel, and the other tiles are solved against it.
 // A wider supernode first applies all descendant updates to every tile of its
 // lower trapezoid in a single level (fully 2D-parallel), then factors itself
-// right-looking, one level per panel, over blocks of RLB panels: within a block
+// right-looking, one level per panel, over blocks of rlb panels: within a block
 // each factored panel updates the block's later panels; after the block's last
 // panel the block updates all later panels with one wide inner dimension.
 #pragma once
@@ -20,9 +20,21 @@ namespace swc {
 
 constexpr int PW = 64;  // panel width (columns)
 constexpr int TM = 64;  // tile height (rows)
-constexpr int RLB = 8;  // panels per block of a wide supernode's right-looking steps
-constexpr int SPW = 128;       // solve panel width (columns)
+constexpr int SPW = 128;       // widest solve panel the solve kernels take (columns)
 constexpr int SPW_NARROW = 16;  // solve panels this narrow run one wave each
+
+// Shape parameters of a plan, chosen per structure class by the autotuner's
+// table (swcholesky_amd/tuning.json, written by swcholesky_amd/tune.py).
+struct PlanParams {
+  int rlb = 8;                 // panels per block of a wide supernode's right-looking steps
+  int spw = SPW;               // solve panel width: 64 or 128 columns
+  int narrow_min_tiles = 1024; // a level runs its narrow tiles one wave each (own launch)
+  int narrow_max_k = 256;      // ... if it has this many; narrow = panel <= 16 columns and
+                               // total inner dimension <= narrow_max_k
+};
+constexpr int kPlanParams = 4;  // entries of the int[] form {rlb, spw, narrow_min_tiles, narrow_max_k}
+// Checks and copies the int[] form; returns false if a value is out of range.
+bool plan_params_from(const int* v, int count, PlanParams& out);
 
 struct Panel {
   int64_t vbase;  // value offset of (panel row 0, panel column 0)
@@ -137,7 +149,7 @@ struct Plan {
 // blocks and forward gathers) is built; the factor lists stay empty.
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
                 const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P,
-                bool factor = true);
+                bool factor = true, const PlanParams& params = PlanParams());
 
 // Descendant sets for a structure given without them: supernode s depends on
 // every d < s whose rows below its own columns meet s's columns, listed in

@@ -2,6 +2,7 @@
 import numpy as np
 import pytest
 
+import swcholesky_amd as S
 from swcholesky_amd import api, problems as P
 import emulate as E
 import oracle_py as O
@@ -145,3 +146,37 @@ def test_derived_descendant_sets_give_the_reference_solve_schedule(name):
     xe = E.solve(der, b.s, Lo, y)
     xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
     assert relF(xe, xo) < 1e-12
+
+
+@pytest.mark.parametrize("params", [(2, 128, 1024, 256), (16, 64, 1024, 256), (8, 64, 1, 512),
+                                    (1, 128, 1, 0), (64, 64, 4096, 128)])
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(12), lambda: P.dense_spd(300)])
+def test_plan_shape_parameters_keep_parity(params, make):
+    """Every plan shape the autotuner may pick (right-looking block size, solve
+    panel width, one-wave tile thresholds) factors and solves to the oracle's
+    result on the emulated kernels."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    plan = api.build_plan(b, params=params)
+    Le, info = E.factor(plan, b.s, b.xsize, _amap(b), a2x)
+    assert info == 0
+    assert relF(Le, Lo) < 1e-12
+    spw = params[1]
+    assert plan["spanels"]["ncol"].max() <= spw
+    y = np.random.default_rng(3).uniform(-1, 1, b.n)
+    xe = E.solve(plan, b.s, Le, y)
+    xo = O.ltsolve(b, Lo, O.lsolve(b, Lo, y))[:, 0]
+    assert relF(xe, xo) < 1e-8  # forward error of a cond ~1e8 operator
+
+
+def test_plan_shape_parameters_are_checked_and_looked_up():
+    from swcholesky_amd import tuning
+    prob = P.laplacian2d_var(37, 3)
+    b, *_ = setup(prob)
+    for bad in [(0,), (8, 96), (8, 128, 0), (8, 128, 16, -1)]:
+        with pytest.raises(S.SwcError):
+            api.build_plan(b, params=bad)
+    entries = [{"class": {"n": 262144, "max_width": 6593}, "params": dict(zip(tuning.NAMES, (4, 64, 256, 128)))},
+               {"class": {"n": 1369, "max_width": 40}, "params": dict(zip(tuning.NAMES, (16, 128, 4096, 512)))}]
+    assert tuning.params_for(b, entries) == (16, 128, 4096, 512)
+    assert tuning.params_for(b, []) == tuning.DEFAULTS
