@@ -99,10 +99,12 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="factor steps only (for counter runs): no refinement paths, no "
                          "per-launch event pass, no CPU baseline")
-    ap.add_argument("--workload", default="lap3d", choices=["lap3d", "vec3d", "batch", "gmrf"],
+    ap.add_argument("--workload", default="lap3d",
+                    choices=["lap3d", "vec3d", "batch", "gmrf", "schedules"],
                     help="lap3d: the headline 64^3 operator (default); vec3d: 48^3 x 3-dof "
                          "operator, mixed precision with 1/8/32 RHS; batch: 4096 uneven 2D "
-                         "instances in one launch sequence; gmrf: 256^2 GMRF gradient")
+                         "instances in one launch sequence; gmrf: 256^2 GMRF gradient; "
+                         "schedules: wavefront vs the reference's H-level schedule")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device-to-device copy of this many bytes (a plain "
                          "copy kernel of known size, to scale HBM counters)")
@@ -380,7 +382,52 @@ def bench_gmrf(args):
                       "higher_is_better": False, "details": res}), flush=True)
 
 
-OTHER = {"vec3d": bench_vec3d, "batch": bench_batch, "gmrf": bench_gmrf}
+def bench_schedules(args):
+    """The wavefront schedule (factor levels from the descendant sets, as the
+    level sets of the supernodal elimination tree) against the reference's
+    H-level schedule (its own clusters and partitions from analyze_p2, stored in
+    tests/golden/full_*.npz): FP64 factor and one-RHS solve on 64^3 and on the
+    48^3 x 3 vector operator."""
+    import torch
+    import swcholesky_amd as S
+    from swcholesky_amd import problems as P
+    res = {}
+    for name, make in (("lap3d_var_64", lambda: P.laplacian3d_var(64)),
+                       ("vec3d_var_48", lambda: P.vector3d_var(48))):
+        g = np.load(os.path.join(ROOT, "tests", "golden", f"full_{name}.npz"))
+        prob = make()
+        b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+        dvals = torch.from_numpy(prob.values).cuda()
+        drhs = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, prob.n)).cuda()
+        for sched_name, sched in (("wavefront", None),
+                                  ("hlevel", (g["levelptr"], g["parptr"], g["partition"]))):
+            f = S.Factor(b, schedule=sched)
+            assert f.factor(dvals) == 0
+            x = drhs.clone()
+            f.solve_device(x)
+            tf, ts = [], []
+            for _ in range(5):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                f.factor(dvals)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                f.solve_device(x)
+                torch.cuda.synchronize()
+                tf.append((t1 - t0) * 1e3)
+                ts.append((time.perf_counter() - t1) * 1e3)
+            rec = {"levels": f.plan_stats()["levels"], "factor_ms": float(np.median(tf)),
+                   "solve_ms": float(np.median(ts)),
+                   "factor_gflops": b.flops / (float(np.median(tf)) * 1e-3) / 1e9}
+            res[f"{name}/{sched_name}"] = rec
+            log(f"{name} {sched_name}: {rec}")
+            del f
+    print(json.dumps({"metric": "FP64 factor time, wavefront vs H-level schedule", "unit": "ms",
+                      "results": res}), flush=True)
+
+
+OTHER = {"vec3d": bench_vec3d, "batch": bench_batch, "gmrf": bench_gmrf,
+         "schedules": bench_schedules}
 
 if __name__ == "__main__":
     main()

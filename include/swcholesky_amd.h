@@ -131,6 +131,19 @@ int swc_set_perm(swc_handle* h, const int* perm);
  * autotuner's table (swcholesky_amd/tuning.json). */
 int swc_set_plan_params(swc_handle* h, const int* params, int count);
 
+/* Schedule of the factor levels of the next structure upload.  nLevels = 0 (the
+ * default): the wavefront schedule, every supernode at the first level its pruned
+ * descendants allow (the level sets of the supernodal elimination tree,
+ * common/TreeUtils.h:119-170).  Otherwise the reference's H-level schedule as
+ * analyze_p2 returns it (InspectionLevel_06.h:18-391; levelPtr[nLevels+1],
+ * parPtr[levelPtr[nLevels]+1], partition[nsuper], the arguments the reference
+ * passes to sw_cholesky_left_par_05): clusters one after another, the partitions
+ * of a cluster side by side, each partition's supernodes one after another.
+ * Offered to compare the two; the upload fails if it does not cover the
+ * structure or places a supernode before its descendants. */
+int swc_set_schedule(swc_handle* h, int nLevels, const int* levelPtr, const int* parPtr,
+                     const int* partition);
+
 /* Precision of the next swc_factor calls.  0 (default): FP64 throughout.
  * 1: descendant SYRK/GEMM updates on FP32 MFMA with FP32 accumulation, each
  *    update subtracted into the FP64 target; POTRF, TRSM and the updates inside
@@ -222,10 +235,12 @@ typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out);
-/* The same with plan shape parameters (as swc_set_plan_params). */
+/* The same with plan shape parameters (as swc_set_plan_params) and, with
+ * nLevels > 0, the H-level schedule (as swc_set_schedule). */
 int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                           const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
-                          const int* params, int nparams, swc_plan** out);
+                          const int* params, int nparams, int nLevels, const int* levelPtr,
+                          const int* parPtr, const int* partition, swc_plan** out);
 void swc_plan_free(swc_plan* p);
 int swc_plan_sizes(const swc_plan* p, int64_t* sizes, double* flops);
 int swc_plan_export(const swc_plan* p, int which, void* dst);

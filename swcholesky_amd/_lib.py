@@ -49,13 +49,14 @@ SIGNATURES = {
     "swc_gradient": (c_int, [c_vp, c_vp, c_int, c_dbl, c_dbl, c_vp, c_int]),
     "swc_set_precision": (c_int, [c_vp, c_int]),
     "swc_set_plan_params": (c_int, [c_vp, c_vp, c_int]),
+    "swc_set_schedule": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "swc_refine": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_int, c_dbl, c_int, c_int, c_vp, c_ip]),
     "swc_residual": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_int, c_vp]),
     "swc_get_timings": (c_int, [c_vp, c_vp]),
     "swc_plan_build": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                ctypes.POINTER(c_vp)]),
     "swc_plan_build_params": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int,
-                                      ctypes.POINTER(c_vp)]),
+                                      c_int, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "swc_plan_free": (None, [c_vp]),
     "swc_plan_sizes": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_plan_export": (c_int, [c_vp, c_int, c_vp]),

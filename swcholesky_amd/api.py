@@ -148,10 +148,12 @@ class Factor:
     solves A x = b in the original ordering.
     """
 
-    def __init__(self, bcsc: BCSC, params="auto"):
+    def __init__(self, bcsc: BCSC, params="auto", schedule=None):
         """params: "auto" (the autotuner's table entry for this structure's class,
         swcholesky_amd/tuning.py), None (the library defaults) or a sequence
-        {rlb, spw, narrow_min_tiles, narrow_max_k} (swc_set_plan_params)."""
+        {rlb, spw, narrow_min_tiles, narrow_max_k} (swc_set_plan_params).
+        schedule: None (wavefront levels) or the reference's H-level arrays
+        (levelPtr, parPtr, partition) (swc_set_schedule)."""
         self.bcsc = bcsc
         self.n = bcsc.n
         lib = _lib.lib()
@@ -166,6 +168,9 @@ class Factor:
         if self.params is not None:
             pv = np.asarray(self.params, dtype=np.int32)
             check(lib.swc_set_plan_params(self._h, ptr(pv), len(pv)))
+        if schedule is not None:
+            lp, pp, part = (np.ascontiguousarray(a, dtype=np.int32) for a in schedule)
+            check(lib.swc_set_schedule(self._h, len(lp) - 1, ptr(lp), ptr(pp), ptr(part)))
         check(lib.swc_upload_symbolic(self._h, bcsc._h))
         self.info = None
 
@@ -314,11 +319,12 @@ SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("writer", np.int32)])
 
 
-def build_plan(bcsc: BCSC, derive_sets=False, params=None):
+def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
     """The device work plan for a structure, as numpy record arrays (host only).
     derive_sets: ignore the analysis's descendant sets and derive them from the
     row lists and build the solve part only (what blockedLsolve does); params:
-    plan shape parameters as swc_set_plan_params takes them."""
+    plan shape parameters as swc_set_plan_params takes them; schedule: the
+    H-level arrays (levelPtr, parPtr, partition) as swc_set_schedule takes them."""
     lib = _lib.lib()
     sizes = np.zeros(4, dtype=np.int64)
     check(lib.swc_plan_struct_sizes(ptr(sizes)))
@@ -331,9 +337,14 @@ def build_plan(bcsc: BCSC, derive_sets=False, params=None):
     iptr = np.ascontiguousarray(bcsc.i_ptr.astype(np.uint64))
     pp, ps = (None, None) if derive_sets else (bcsc.prunePtr, bcsc.pruneSet)
     pv = None if params is None else np.asarray(params, dtype=np.int32)
+    lp = pr = part = None
+    if schedule is not None:
+        lp, pr, part = (np.ascontiguousarray(a, dtype=np.int32) for a in schedule)
     check(lib.swc_plan_build_params(bcsc.n, ptr(super_), bcsc.nsuper, ptr(p), ptr(bcsc.s),
                                     ptr(iptr), ptr(pp), ptr(ps), ptr(pv),
-                                    0 if pv is None else len(pv), ctypes.byref(h)))
+                                    0 if pv is None else len(pv),
+                                    0 if lp is None else len(lp) - 1, ptr(lp), ptr(pr), ptr(part),
+                                    ctypes.byref(h)))
     try:
         sz = np.zeros(12, dtype=np.int64)
         fl = (ctypes.c_double * 2)()

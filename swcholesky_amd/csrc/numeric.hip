@@ -1559,6 +1559,7 @@ struct swc_handle {
   hipStream_t stream = nullptr;
   Plan plan;
   swc::PlanParams params;  // shape parameters of the next structure upload
+  swc::HLevelSchedule hlevel;  // non-empty: factor levels follow this schedule
   int n = 0, nsuper = 0;
   int64_t nvals = 0;  // number of values swc_factor expects
   std::vector<int> super, Ls;
@@ -2315,7 +2316,8 @@ int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const
   h->pruneset.assign(pruneSet, pruneSet + prunePtr[supNo]);
   try {
     swc::build_plan(n, supNo, h->super.data(), h->pi.data(), h->px.data(), h->Ls.data(),
-                    prunePtr, pruneSet, h->plan, factor, h->params);
+                    prunePtr, pruneSet, h->plan, factor, h->params,
+                    h->hlevel.level_ptr.empty() ? nullptr : &h->hlevel);
   } catch (const std::bad_alloc&) {
     return swc_set_error(SWC_ENOMEM, "upload: out of host memory");
   } catch (const std::exception& e) {
@@ -2653,6 +2655,26 @@ int swc_set_plan_params(swc_handle* h, const int* params, int count) {
   if (params && !swc::plan_params_from(params, count, p))
     return swc_set_error(SWC_EINVAL, "swc_set_plan_params: parameter out of range");
   h->params = p;  // NULL: the defaults
+  return SWC_OK;
+}
+
+int swc_set_schedule(swc_handle* h, int nLevels, const int* levelPtr, const int* parPtr,
+                     const int* partition) {
+  if (int rc = check_handle(h)) return rc;
+  if (nLevels <= 0 || !levelPtr) {
+    h->hlevel = swc::HLevelSchedule();
+    return SWC_OK;
+  }
+  if (!parPtr || !partition || levelPtr[0] != 0 || levelPtr[nLevels] < 0)
+    return swc_set_error(SWC_EINVAL, "swc_set_schedule: bad argument");
+  const int npar = levelPtr[nLevels];
+  if (parPtr[0] != 0 || parPtr[npar] < 0)
+    return swc_set_error(SWC_EINVAL, "swc_set_schedule: bad partition pointers");
+  swc::HLevelSchedule hl;
+  hl.level_ptr.assign(levelPtr, levelPtr + nLevels + 1);
+  hl.par_ptr.assign(parPtr, parPtr + npar + 1);
+  hl.partition.assign(partition, partition + parPtr[npar]);
+  h->hlevel = std::move(hl);  // checked against the structure at the next upload
   return SWC_OK;
 }
 

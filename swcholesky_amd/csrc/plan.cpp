@@ -71,7 +71,7 @@ bool plan_params_from(const int* v, int count, PlanParams& out) {
 
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
                 const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P, bool factor,
-                const PlanParams& params) {
+                const PlanParams& params, const HLevelSchedule* hl) {
   const int RLB = params.rlb, SPWP = params.spw;
   P = Plan();
   P.n = n;
@@ -228,10 +228,16 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     ptasks[ti].push_back(t);
   };
 
-  for (int s = 0; s < nsuper && factor; ++s) {
+  // Supernode s starts at the first level after its pruned descendants complete
+  // and not before `floor`; returns the level at which it completes.
+  auto place = [&](int s, int floor) {
     const int fp = first_panel[s], lp = first_panel[s + 1], np = lp - fp;
-    int le = 0;
-    for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) le = std::max(le, sn_level[pruneSet[t]] + 1);
+    int le = floor;
+    for (int t = prunePtr[s]; t < prunePtr[s + 1]; ++t) {
+      if (sn_level[pruneSet[t]] < 0)
+        throw std::invalid_argument("schedule places a supernode before its descendants");
+      le = std::max(le, sn_level[pruneSet[t]] + 1);
+    }
     // external phase (and, for one panel, the whole factorization) at level le
     for (int J = 0; J < np; ++J) {
       const Panel& pn = P.panels[fp + J];
@@ -267,6 +273,39 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       P.panel_level[fp + p + 1] = lvl;
     }
     sn_level[s] = le + np - 1;
+    return sn_level[s];
+  };
+  if (factor && !hl) {
+    // wavefront: every supernode as early as its descendants allow (the level
+    // sets of the supernodal elimination tree, TreeUtils.h:119-170)
+    for (int s = 0; s < nsuper; ++s) place(s, 0);
+  } else if (factor) {
+    // the reference's H-level schedule: clusters one after another (a barrier
+    // between them), the partitions of a cluster side by side, each partition's
+    // supernodes one after another in its order (sw_parallel_PB_Cholesky_05.c:
+    // the level-cluster loop, one thread per partition)
+    const int ncl = (int)hl->level_ptr.size() - 1;
+    if (ncl < 0 || hl->par_ptr.empty() || hl->level_ptr[0] != 0 ||
+        hl->level_ptr[ncl] != (int)hl->par_ptr.size() - 1 || hl->par_ptr[0] != 0 ||
+        hl->par_ptr.back() != nsuper || (int)hl->partition.size() != nsuper)
+      throw std::invalid_argument("H-level schedule does not cover the supernodes");
+    std::vector<char> seen(nsuper, 0);
+    int done = -1;  // last level of the clusters placed so far
+    for (int k = 0; k < ncl; ++k) {
+      int cl_end = done;
+      for (int q = hl->level_ptr[k]; q < hl->level_ptr[k + 1]; ++q) {
+        int prev = done;
+        for (int i = hl->par_ptr[q]; i < hl->par_ptr[q + 1]; ++i) {
+          const int sn = hl->partition[i];
+          if (sn < 0 || sn >= nsuper || seen[sn])
+            throw std::invalid_argument("H-level schedule lists a supernode twice or out of range");
+          seen[sn] = 1;
+          prev = place(sn, prev + 1);
+          cl_end = std::max(cl_end, prev);
+        }
+      }
+      done = cl_end;
+    }
   }
   P.nlevel = 0;
   for (const PTile& pt : ptiles) P.nlevel = std::max(P.nlevel, pt.level + 1);
@@ -519,15 +558,25 @@ int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, cons
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out) {
   return swc_plan_build_params(n, blockSet, supNo, lC, Ls, Li_ptr, prunePtr, pruneSet, nullptr, 0,
-                               out);
+                               0, nullptr, nullptr, nullptr, out);
 }
 
 int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                           const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
-                          const int* params, int nparams, swc_plan** out) {
+                          const int* params, int nparams, int nLevels, const int* levelPtr,
+                          const int* parPtr, const int* partition, swc_plan** out) {
   swc::PlanParams prm;
   if (params && !swc::plan_params_from(params, nparams, prm))
     return swc_set_error(SWC_EINVAL, "swc_plan_build: plan parameter out of range");
+  swc::HLevelSchedule hl;
+  if (nLevels > 0) {
+    if (!levelPtr || !parPtr || !partition || levelPtr[0] != 0 || levelPtr[nLevels] < 0 ||
+        parPtr[0] != 0 || parPtr[levelPtr[nLevels]] < 0)
+      return swc_set_error(SWC_EINVAL, "swc_plan_build: bad H-level schedule");
+    hl.level_ptr.assign(levelPtr, levelPtr + nLevels + 1);
+    hl.par_ptr.assign(parPtr, parPtr + levelPtr[nLevels] + 1);
+    hl.partition.assign(partition, partition + parPtr[levelPtr[nLevels]]);
+  }
   // pruneSet may be null when it is empty (no supernode has descendants); with
   // prunePtr null both are derived from the row lists and only the solve part of
   // the plan is built (what blockedLsolve does)
@@ -548,7 +597,7 @@ int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* l
     }
     auto* h = new swc_plan;
     swc::build_plan(n, supNo, blockSet, pi.data(), px.data(), Ls, prunePtr, pruneSet, h->P, factor,
-                    prm);
+                    prm, nLevels > 0 ? &hl : nullptr);
     *out = h;
     return SWC_OK;
   } catch (const std::exception& e) {

@@ -147,9 +147,19 @@ struct Plan {
 // Ls row indices, prunePtr/pruneSet as the reference's analysis produces them.
 // With factor = false only the solve part (solve panels, their levels, row
 // blocks and forward gathers) is built; the factor lists stay empty.
+// The reference's H-level schedule (getCoarseLevelSet_6, InspectionLevel_06.h:
+// 18-391): clusters [level_ptr[k], level_ptr[k+1]) of partitions, partition q the
+// supernodes partition[par_ptr[q] .. par_ptr[q+1]) in execution order.
+struct HLevelSchedule {
+  std::vector<int> level_ptr, par_ptr, partition;
+};
+
+// hl == nullptr: the wavefront schedule (each supernode at the first level its
+// descendants allow); otherwise the levels follow hl (for comparison).
 void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const int64_t* px,
                 const int* Ls, const int* prunePtr, const int* pruneSet, Plan& P,
-                bool factor = true, const PlanParams& params = PlanParams());
+                bool factor = true, const PlanParams& params = PlanParams(),
+                const HLevelSchedule* hl = nullptr);
 
 // Descendant sets for a structure given without them: supernode s depends on
 // every d < s whose rows below its own columns meet s's columns, listed in

@@ -91,3 +91,23 @@ def test_bad_perm_rejected():
     perm[0] = perm[1]
     with pytest.raises(S.SwcError):
         S.analyze_p2(prob.n, prob.colptr, prob.rowidx, perm)
+
+
+@pytest.mark.parametrize("name", ["lap3d_var_64", "vec3d_var_48"])
+def test_full_size_analysis_matches_reference_digests(name):
+    """At full size (n = 262,144 and 331,776) the restated analysis equals the
+    reference's analyze_p2 array for array: SHA-256 digests of the reference's
+    outputs, written by tests/golden/make_golden.py from oracle/_ref."""
+    import hashlib
+    import os
+    import swcholesky_amd as S
+    from swcholesky_amd import problems as P
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"full_{name}.npz"))
+    prob = {"lap3d_var_64": lambda: P.laplacian3d_var(64), "vec3d_var_48": lambda: P.vector3d_var(48)}[name]()
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    ours = {"perm": b.Perm, "colcount": b.ColCount, "super": b.super, "s": b.s,
+            "pruneptr": b.prunePtr, "pruneset": b.pruneSet, "p": b.p, "i_ptr": b.i_ptr}
+    for key, dig, dt in zip(g["digest_keys"], g["digests"], g["dtypes"]):
+        a = np.ascontiguousarray(ours[str(key)].astype(str(dt)))
+        assert hashlib.sha256(a.tobytes()).hexdigest() == str(dig), key
+    assert int(g["header"][1]) == b.nsuper

@@ -261,3 +261,23 @@ def test_blockedLsolve_drop_in(name, make):
     assert np.max(np.abs(y - 1.0)) <= 4 * err_o + 1e-14
     res = np.max(np.abs(Ld @ y - rhs)) / (abs(Ld).sum(axis=1).max() * np.max(np.abs(y)))
     assert res <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["c1_lap2d_const_100", "lap3d_var_12", "vec3d_var_6"])
+def test_hlevel_schedule_matches_oracle(name):
+    """Factor levels from the reference's own H-level schedule (golden fixture
+    of its analyze_p2): the same factor as the oracle, and as the wavefront
+    schedule's within rounding."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    prob = {"c1_lap2d_const_100": lambda: P.laplacian2d_const(100),
+            "lap3d_var_12": lambda: P.laplacian3d_var(12),
+            "vec3d_var_6": lambda: P.vector3d_var(6)}[name]()
+    b, a2x, Lo, _ = setup(prob)
+    f = S.Factor(b, schedule=(g["levelptr"], g["parptr"], g["partition"]))
+    assert f.factor(prob.values) == 0
+    assert relF(f.L_values(), Lo) <= 1e-12
+    A, rp, ci, v = full_csr(prob)
+    rhs = A @ np.random.default_rng(8).uniform(-1, 1, prob.n)
+    x = f.solve(rhs)
+    assert O.backward_error(rp, ci, v, x, rhs) <= 1e-14

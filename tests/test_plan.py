@@ -180,3 +180,41 @@ def test_plan_shape_parameters_are_checked_and_looked_up():
                {"class": {"n": 1369, "max_width": 40}, "params": dict(zip(tuning.NAMES, (16, 128, 4096, 512)))}]
     assert tuning.params_for(b, entries) == (16, 128, 4096, 512)
     assert tuning.params_for(b, []) == tuning.DEFAULTS
+
+
+@pytest.mark.parametrize("name", ["c1_lap2d_const_100", "gmrf2d_32", "lap3d_var_12", "vec3d_var_6"])
+def test_hlevel_schedule_plan(name):
+    """The reference's H-level schedule (its own clusters and partitions, golden
+    fixtures from its analyze_p2) as factor levels: clusters one after another,
+    each partition's supernodes in order.  The emulated kernels on that plan give
+    the oracle's factor, every supernode starts after its descendants and after
+    its partition predecessor, and the wavefront plan never has more levels."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    prob = {"c1_lap2d_const_100": lambda: P.laplacian2d_const(100),
+            "gmrf2d_32": lambda: P.gmrf2d(32), "lap3d_var_12": lambda: P.laplacian3d_var(12),
+            "vec3d_var_6": lambda: P.vector3d_var(6)}[name]()
+    b, a2x, Lo, _ = setup(prob)
+    sched = (g["levelptr"], g["parptr"], g["partition"])
+    hl = api.build_plan(b, schedule=sched)
+    wf = api.build_plan(b)
+    assert wf["nlevel"] <= hl["nlevel"]
+    Le, info = E.factor(hl, b.s, b.xsize, _amap(b), a2x)
+    assert info == 0 and relF(Le, Lo) < 1e-12
+    # level of each supernode's first panel, and of its last
+    first = {}
+    last = {}
+    for lvl in range(hl["nlevel"]):
+        for q in hl["panel_order"][hl["lvl_panels"][lvl]:hl["lvl_panels"][lvl + 1]]:
+            sn = int(hl["panels"][q]["sn"])
+            first[sn] = min(first.get(sn, lvl), lvl)
+            last[sn] = max(last.get(sn, lvl), lvl)
+    lp, pr, part = sched
+    for q in range(len(pr) - 1):
+        seq = part[pr[q]:pr[q + 1]]
+        for a, c in zip(seq[:-1], seq[1:]):
+            assert first[int(c)] > last[int(a)]
+    for k in range(1, len(lp) - 1):
+        prev = [int(s) for q in range(lp[k - 1], lp[k]) for s in part[pr[q]:pr[q + 1]]]
+        cur = [int(s) for q in range(lp[k], lp[k + 1]) for s in part[pr[q]:pr[q + 1]]]
+        assert min(first[s] for s in cur) > max(last[s] for s in prev)
