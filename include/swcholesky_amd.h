@@ -99,6 +99,14 @@ int sw_cholesky_left_par_05(int n, int* c, int* r, double* values, size_t* lC, i
 int blockedLsolve(int n, size_t* Lp, int* Li, double* Lx, int NNZ, size_t* Li_ptr, int* col2sup,
                   int* sup2col, int supNo, double* x);
 
+/* ---- reference CSC export -------------------------------------------------
+ * Drop-in for bcsc2csc (common/Util.h:311-338), host arrays in and out, MKL_INT
+ * as a 32-bit int (the reference's LP64 build): the BCSC factor (Ap = L->p, Ai =
+ * L->s, AiP = L->i_ptr, sup2col = L->super, Ax = values) to CSC Cp[n+1], Ci, Cx.
+ * Returns 1, or 0 on a bad argument or a factor with more than 2^31-1 entries. */
+int bcsc2csc(size_t n, size_t nBlocks, size_t* Ap, int* Ai, size_t* AiP, int* sup2col, double* Ax,
+             int* Cp, int* Ci, double* Cx);
+
 /* ---- handle API (structure uploaded once, refactor / solve many times) ------ */
 typedef struct swc_handle swc_handle;
 
@@ -158,6 +166,13 @@ int swc_set_precision(swc_handle* h, int mode);
 int swc_factor(swc_handle* h, const double* values, int values_on_device, int* info);
 /* Copies L (xsize doubles, BCSC layout) to host or device memory. */
 int swc_get_factor(swc_handle* h, double* lValues, int on_device);
+/* The factor in CSC, as the reference's bcsc2csc lays it out (common/Util.h:
+ * 311-338): column j holds its supernode's rows from its own diagonal down
+ * (explicit zeros of relaxed amalgamation included), rows ascending.
+ * swc_factor_nnz gives the entry count; swc_factor_csc fills Cp[n+1] (int64),
+ * Ci[nnz], Cx[nnz] in host or device memory, converting on the GPU. */
+int swc_factor_nnz(swc_handle* h, int64_t* nnz);
+int swc_factor_csc(swc_handle* h, int64_t* Cp, int* Ci, double* Cx, int on_device);
 /* Solves A X = B in place for nrhs columns (leading dimension ldb >= n):
  * permutation, forward L y = P b (blockedLsolve, triangularSolve/Triangular_BCSC.h:14-49),
  * backward L^T z = y (no reference counterpart), inverse permutation. */

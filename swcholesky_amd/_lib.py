@@ -38,6 +38,10 @@ SIGNATURES = {
     "swc_set_perm": (c_int, [c_vp, c_vp]),
     "swc_factor": (c_int, [c_vp, c_vp, c_int, c_ip]),
     "swc_get_factor": (c_int, [c_vp, c_vp, c_int]),
+    "swc_factor_nnz": (c_int, [c_vp, ctypes.POINTER(c_i64)]),
+    "swc_factor_csc": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int]),
+    "bcsc2csc": (c_int, [ctypes.c_size_t, ctypes.c_size_t, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                         c_vp]),
     "swc_solve": (c_int, [c_vp, c_vp, c_int, c_i64, c_int]),
     "swc_solve_noalloc": (c_int, [c_vp, c_vp, c_int, c_i64]),
     "swc_plan_stats": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),

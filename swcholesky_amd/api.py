@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import check, ptr
+from ._lib import SwcError, check, ptr
 
 _SYM = {
     "Perm": (0, "n", np.int32), "ColCount": (1, "n", np.int32), "parent": (2, "n", np.int32),
@@ -131,6 +131,26 @@ def blockedLsolve(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x):
                                     ptr(col2sup), ptr(sup2col), int(supNo), ptr(x))
 
 
+def bcsc2csc(bcsc, values):
+    """The reference's bcsc2csc (common/Util.h:311-338) on host arrays: returns
+    (Cp, Ci, Cx) with 32-bit indices."""
+    n = bcsc.n
+    nnz = int(sum((bcsc.pi[s + 1] - bcsc.pi[s]) * (bcsc.super[s + 1] - bcsc.super[s])
+                  - (bcsc.super[s + 1] - bcsc.super[s]) * (bcsc.super[s + 1] - bcsc.super[s] - 1) // 2
+                  for s in range(bcsc.nsuper)))
+    Cp = np.zeros(n + 1, dtype=np.int32)
+    Ci = np.zeros(max(nnz, 1), dtype=np.int32)
+    Cx = np.zeros(max(nnz, 1), dtype=np.float64)
+    Ap = np.ascontiguousarray(bcsc.p, dtype=np.uint64)
+    AiP = np.ascontiguousarray(bcsc.i_ptr, dtype=np.uint64)
+    Ax = np.ascontiguousarray(values, dtype=np.float64)
+    ok = _lib.lib().bcsc2csc(n, bcsc.nsuper, ptr(Ap), ptr(bcsc.s), ptr(AiP), ptr(bcsc.super),
+                             ptr(Ax), ptr(Cp), ptr(Ci), ptr(Cx))
+    if ok != 1:
+        raise SwcError(-1, _lib.lib().swc_last_error().decode(errors="replace"))
+    return Cp, Ci[:nnz], Cx[:nnz]
+
+
 def _dptr(x):
     """(pointer, on_device) of a numpy array or a torch tensor."""
     if isinstance(x, np.ndarray):
@@ -195,6 +215,19 @@ class Factor:
         out = np.zeros(self.bcsc.xsize, dtype=np.float64)
         check(_lib.lib().swc_get_factor(self._h, ptr(out), 0))
         return out
+
+    def L_csc(self):
+        """The factor as a scipy CSC matrix in the layout of the reference's
+        bcsc2csc, converted on the GPU (swc_factor_csc)."""
+        import scipy.sparse as sp
+        lib = _lib.lib()
+        nnz = ctypes.c_int64()
+        check(lib.swc_factor_nnz(self._h, ctypes.byref(nnz)))
+        Cp = np.zeros(self.n + 1, dtype=np.int64)
+        Ci = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        Cx = np.zeros(max(nnz.value, 1), dtype=np.float64)
+        check(lib.swc_factor_csc(self._h, ptr(Cp), ptr(Ci), ptr(Cx), 0))
+        return sp.csc_matrix((Cx[:nnz.value], Ci[:nnz.value], Cp), shape=(self.n, self.n))
 
     def solve(self, b):
         """x = A^{-1} b for a vector or an n x nrhs matrix (numpy, returns a copy)."""

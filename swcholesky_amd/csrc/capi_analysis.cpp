@@ -189,4 +189,39 @@ int swc_symbolic_copy(const swc_symbolic* sym, int which, void* dst) {
   }
 }
 
+// Reference export of the factor to CSC (common/Util.h:311-338, MKL_INT as a
+// 32-bit int, the reference's LP64 build): column j of supernode [c0, c1) takes
+// the rows of the supernode's row list from its own diagonal down, with their
+// stored values (explicit zeros of relaxed amalgamation included).
+int bcsc2csc(size_t n, size_t nBlocks, size_t* Ap, int* Ai, size_t* AiP, int* sup2col, double* Ax,
+             int* Cp, int* Ci, double* Cx) {
+  if (!Ap || !Ai || !AiP || !sup2col || !Ax || !Cp || !Ci || !Cx) {
+    swc_set_error(SWC_EINVAL, "bcsc2csc: null argument");
+    return 0;
+  }
+  int64_t nnz = 0;
+  Cp[0] = 0;
+  for (size_t b = 0; b < nBlocks; ++b) {
+    const int c0 = sup2col[b], c1 = sup2col[b + 1];
+    if (c1 <= c0 || (size_t)c1 > n) {
+      swc_set_error(SWC_EINVAL, "bcsc2csc: bad supernode partition");
+      return 0;
+    }
+    for (int j = c0; j < c1; ++j) {
+      size_t kk = AiP[c0] + (size_t)(j - c0);
+      for (size_t k = Ap[j] + (size_t)(j - c0); k < Ap[j + 1]; ++k, ++kk) {
+        if (Ai[kk] < 0 || (size_t)Ai[kk] >= n || nnz >= 0x7fffffffLL) {
+          swc_set_error(SWC_EINVAL, "bcsc2csc: row index out of range or factor too large");
+          return 0;
+        }
+        Cx[nnz] = Ax[k];
+        Ci[nnz] = Ai[kk];
+        ++nnz;
+      }
+      Cp[j + 1] = (int)nnz;
+    }
+  }
+  return 1;
+}
+
 }  // extern "C"

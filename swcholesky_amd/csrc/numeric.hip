@@ -583,6 +583,30 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
   }
 }
 
+// CSC export of the factor (the layout of the reference's bcsc2csc,
+// common/Util.h:311-338): column j's entries are its supernode's rows from its
+// own diagonal down.  col[j] = {value offset of L(j, j), offset in Ls of row j,
+// entry count, offset in the output}; one wave per column.
+struct CscColumn {
+  int64_t v0, r0, len, c0;
+};
+
+__global__ void k_factor_csc(const double* __restrict__ L, const int* __restrict__ Ls,
+                             const CscColumn* __restrict__ col, int n, int64_t* __restrict__ Cp,
+                             int* __restrict__ Ci, double* __restrict__ Cx) {
+  const int j = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= n) return;
+  const CscColumn c = col[j];
+  for (int64_t e = lane; e < c.len; e += 64) {
+    Cx[c.c0 + e] = L[c.v0 + e];
+    Ci[c.c0 + e] = Ls[c.r0 + e];
+  }
+  if (lane == 0) {
+    Cp[j + 1] = c.c0 + c.len;
+    if (j == 0) Cp[0] = 0;
+  }
+}
+
 // ---- triangular solves --------------------------------------------------
 // Forward L y = b (blockedLsolve, Triangular_BCSC.h:14-49, restated level by
 // level).  The reference pushes L21(s) x_s into the rows below each supernode
@@ -2646,6 +2670,75 @@ int swc_residual(swc_handle* h, const double* B, const double* X, int nrhs, int6
     const double den = h->normA * xn[c] + bn[c];
     eta[c] = den > 0 ? rn[c] / den : 0.0;
   }
+  return SWC_OK;
+}
+
+namespace {
+// Per-column CSC descriptors of the uploaded structure and the factor's entry count.
+std::vector<CscColumn> csc_columns(const swc_handle* h, int64_t* nnz) {
+  std::vector<CscColumn> cols(h->n);
+  int64_t off = 0;
+  for (int s = 0; s < h->nsuper; ++s) {
+    const int c0 = h->super[s], c1 = h->super[s + 1];
+    const int64_t R = h->pi[s + 1] - h->pi[s];
+    for (int j = c0; j < c1; ++j) {
+      const int64_t d = j - c0;
+      cols[j] = CscColumn{h->px[s] + d * R + d, h->pi[s] + d, R - d, off};
+      off += R - d;
+    }
+  }
+  *nnz = off;
+  return cols;
+}
+}  // namespace
+
+int swc_factor_nnz(swc_handle* h, int64_t* nnz) {
+  if (int rc = check_handle(h)) return rc;
+  if (!nnz) return swc_set_error(SWC_EINVAL, "swc_factor_nnz: null argument");
+  if (h->n <= 0) return swc_set_error(SWC_ESTATE, "swc_factor_nnz: no structure");
+  (void)csc_columns(h, nnz);
+  return SWC_OK;
+}
+
+int swc_factor_csc(swc_handle* h, int64_t* Cp, int* Ci, double* Cx, int on_device) {
+  if (int rc = check_handle(h)) return rc;
+  if (!Cp || !Ci || !Cx) return swc_set_error(SWC_EINVAL, "swc_factor_csc: null argument");
+  if (!h->factored) return swc_set_error(SWC_ESTATE, "swc_factor_csc: not factored");
+  int64_t nnz = 0;
+  const std::vector<CscColumn> cols = csc_columns(h, &nnz);
+  hipStream_t st = h->stream;
+  CscColumn* dcol = nullptr;
+  int64_t* dCp = on_device ? Cp : nullptr;
+  int* dCi = on_device ? Ci : nullptr;
+  double* dCx = on_device ? Cx : nullptr;
+  hipError_t e = hipMalloc((void**)&dcol, sizeof(CscColumn) * cols.size());
+  if (e == hipSuccess && !on_device) e = hipMalloc((void**)&dCp, sizeof(int64_t) * (h->n + 1));
+  if (e == hipSuccess && !on_device) e = hipMalloc((void**)&dCi, sizeof(int) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess && !on_device) e = hipMalloc((void**)&dCx, sizeof(double) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dcol, cols.data(), sizeof(CscColumn) * cols.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    const int per = kThreads / 64;
+    hipLaunchKernelGGL(k_factor_csc, dim3((h->n + per - 1) / per), dim3(kThreads), 0, st,
+                       (const double*)h->dL, (const int*)h->dLs, (const CscColumn*)dcol, h->n, dCp,
+                       dCi, dCx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && !on_device) {
+    e = hipMemcpyAsync(Cp, dCp, sizeof(int64_t) * (h->n + 1), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(Ci, dCi, sizeof(int) * nnz, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(Cx, dCx, sizeof(double) * nnz, hipMemcpyDeviceToHost, st);
+  }
+  const hipError_t se = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = se;
+  (void)hipFree(dcol);
+  if (!on_device) {
+    (void)hipFree(dCp);
+    (void)hipFree(dCi);
+    (void)hipFree(dCx);
+  }
+  if (e != hipSuccess)
+    return swc_set_error(SWC_EDEVICE, std::string("swc_factor_csc: ") + hipGetErrorString(e));
   return SWC_OK;
 }
 

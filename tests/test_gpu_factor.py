@@ -281,3 +281,26 @@ def test_hlevel_schedule_matches_oracle(name):
     rhs = A @ np.random.default_rng(8).uniform(-1, 1, prob.n)
     x = f.solve(rhs)
     assert O.backward_error(rp, ci, v, x, rhs) <= 1e-14
+
+
+def test_factor_from_mtx_and_perm_files_exported_as_csc(tmp_path):
+    """The reference's file path end to end: the operator written as
+    MatrixMarket lower triangle and a binary size_t perm file, read back,
+    analysed and factored on the GPU, the factor exported to CSC on the GPU
+    (swc_factor_csc) and compared with the host bcsc2csc of the oracle's factor:
+    same pattern, values within 1e-12."""
+    from swcholesky_amd import api
+    prob = P.laplacian3d_var(20)
+    P.write_mtx(str(tmp_path / "a.mtx"), prob.n, prob.colptr, prob.rowidx, prob.values)
+    P.write_perm(str(tmp_path / "a.perm"), prob.perm)
+    n, cp, ri, v = P.read_mtx(str(tmp_path / "a.mtx"))
+    perm = P.read_perm(str(tmp_path / "a.perm"), n)
+    b = S.analyze_p2(n, cp, ri, perm)
+    f = S.Factor(b)
+    assert f.factor(v) == 0
+    Lc = f.L_csc()
+    Lo, info = O.factor(b, b.a2_values(v))
+    assert info == 0
+    Cp, Ci, Cx = api.bcsc2csc(b, Lo)
+    assert np.array_equal(Lc.indptr, Cp) and np.array_equal(Lc.indices, Ci)
+    assert np.linalg.norm(Lc.data - Cx) <= 1e-12 * np.linalg.norm(Cx)

@@ -50,3 +50,51 @@ def test_gmrf_precision_definition():
     A = prob.scipy_full()
     lap = A / 2.0 - 0.5 * np.eye(prob.n)
     assert np.allclose(np.asarray(lap.sum(axis=1)).ravel(), 0)
+
+
+def test_bcsc2csc_export_layout():
+    """bcsc2csc (the reference's export, common/Util.h:311-338) against a
+    column-by-column numpy restatement: column j holds its supernode's rows from
+    its own diagonal down with their stored values; L times its transpose
+    reproduces P A P^T."""
+    import numpy as np
+    import swcholesky_amd as S
+    from swcholesky_amd import api
+    import oracle_py as O
+    prob = P.laplacian3d_var(10)
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    L, info = O.factor(b, b.a2_values(prob.values))
+    assert info == 0
+    Cp, Ci, Cx = api.bcsc2csc(b, L)
+    exp_p, exp_i, exp_x = [0], [], []
+    for s in range(b.nsuper):
+        c0, c1 = int(b.super[s]), int(b.super[s + 1])
+        R = b.s[b.pi[s]:b.pi[s + 1]]
+        for j in range(c0, c1):
+            d = j - c0
+            exp_i.extend(R[d:])
+            exp_x.extend(L[b.p[j] + d:b.p[j + 1]])
+            exp_p.append(len(exp_i))
+    assert np.array_equal(Cp, exp_p) and np.array_equal(Ci, exp_i) and np.array_equal(Cx, exp_x)
+    import scipy.sparse as sp
+    Lc = sp.csc_matrix((Cx, Ci, Cp), shape=(b.n, b.n))
+    A = prob.scipy_full()[b.Perm][:, b.Perm]
+    assert abs(Lc @ Lc.T - A).max() < 1e-10 * abs(A).max()
+
+
+def test_problem_round_trips_through_mtx_and_perm_files(tmp_path):
+    """A problem written as MatrixMarket lower triangle + binary size_t perm (the
+    files the reference reads, Util.h:77-223, sw_choleskyTest01.cpp:79) reads
+    back to the same arrays and the same analysis."""
+    import numpy as np
+    import swcholesky_amd as S
+    prob = P.laplacian3d_var(8)
+    P.write_mtx(str(tmp_path / "a.mtx"), prob.n, prob.colptr, prob.rowidx, prob.values)
+    P.write_perm(str(tmp_path / "a.perm"), prob.perm)
+    n, cp, ri, v = P.read_mtx(str(tmp_path / "a.mtx"))
+    perm = P.read_perm(str(tmp_path / "a.perm"), n)
+    assert n == prob.n and np.array_equal(cp, prob.colptr) and np.array_equal(ri, prob.rowidx)
+    assert np.array_equal(v, prob.values) and np.array_equal(perm, prob.perm)
+    b1 = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    b2 = S.analyze_p2(n, cp, ri, perm)
+    assert np.array_equal(b1.s, b2.s) and np.array_equal(b1.pruneSet, b2.pruneSet)
