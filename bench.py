@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--cpu-side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ordering", default="grid", choices=["grid", "nd"],
+                    help="grid: geometric nested dissection of the grid (the workload's "
+                         "ordering); nd: the library's graph nested dissection (swc_nd_order)")
     ap.add_argument("--quick", action="store_true",
                     help="factor steps only (for counter runs): no refinement paths, no "
                          "per-launch event pass, no CPU baseline")
@@ -123,6 +126,8 @@ def main():
 
     log(f"building operator {args.side}^3")
     prob = P.laplacian3d_var(args.side)
+    if args.ordering == "nd":
+        prob.perm = P.nd_order(prob.n, prob.colptr, prob.rowidx)
     t0 = time.perf_counter()
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     t_an = time.perf_counter() - t0
@@ -243,7 +248,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"lap3d_var_{args.side}: 7-point Laplacian {args.side}^3, "
-                               "weights log-uniform [1e-2,1e2] seed 1, shift 1e-6, ND ordering",
+                               "weights log-uniform [1e-2,1e2] seed 1, shift 1e-6, "
+                               + ("geometric grid ND ordering" if args.ordering == "grid"
+                                  else "graph ND ordering (swc_nd_order)"),
                    "n": prob.n, "nnz_tril": prob.nnz, "nsuper": b.nsuper, "flops_F": b.flops,
                    "plan_params": dict(zip(("rlb", "spw", "narrow_min_tiles", "narrow_max_k"),
                                            f.params or ())),

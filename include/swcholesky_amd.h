@@ -73,6 +73,14 @@ enum swc_symbolic_array {
 };
 int swc_symbolic_copy(const swc_symbolic* sym, int which, void* dst);
 
+/* Fill-reducing nested-dissection ordering of a symmetric pattern given as its
+ * lower triangle in CSC (Ap[n+1], Ai): perm[new] = old (int64), to be passed to
+ * swc_analyze as the GIVEN ordering.  Stands in for the METIS_NodeND call of the
+ * reference's x86 path (cholesky/LSparsity.h:597) and the perm files it reads
+ * (software/README.md:45); the ordering is this library's own heuristic, not
+ * METIS's. */
+int swc_nd_order(int n, const int* Ap, const int* Ai, int64_t* perm);
+
 /* ---- reference numeric entry point ----------------------------------------
  * Drop-in for sw_cholesky_left_par_05 (cholesky/sw_parallel_PB_Cholesky_05.h:52-66,
  * PRUNE variant; called at examples/sw_choleskyTest01.cpp:223-232).  A2 = tril(P A

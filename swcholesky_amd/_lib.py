@@ -24,6 +24,7 @@ SIGNATURES = {
     "swc_analyze_batch": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int,
                                   ctypes.POINTER(c_vp)]),
     "swc_symbolic_free": (None, [c_vp]),
+    "swc_nd_order": (c_int, [c_int, c_vp, c_vp, c_vp]),
     "swc_symbolic_sizes": (c_int, [c_vp, c_vp, ctypes.POINTER(c_dbl)]),
     "swc_symbolic_copy": (c_int, [c_vp, c_int, c_vp]),
     "sw_cholesky_left_par_05": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

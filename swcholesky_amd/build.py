@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libswcholesky_amd.so")
 
 HOST_SOURCES = ["analysis.cpp", "capi_analysis.cpp", "capi_common.cpp"]
 HIP_SOURCES = ["numeric.hip"]
-HOST_SOURCES += ["plan.cpp"]
+HOST_SOURCES += ["plan.cpp", "ordering.cpp"]
 
 # -packed-fp32-ops: keep hipcc from forming packed FP32 instructions (v_pk_*_f32),
 # some forms of which have given wrong results on this hardware.

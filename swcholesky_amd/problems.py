@@ -250,6 +250,44 @@ def grid_nd_order(shape, dof=1, leaf=8):
 # reference file formats
 # ---------------------------------------------------------------------------
 
+def nd_order(n, colptr, rowidx):
+    """Nested-dissection ordering of a general symmetric pattern (lower CSC),
+    ``perm[new] = old``: the library's own graph heuristic (swc_nd_order), for
+    operators without grid coordinates."""
+    import ctypes
+    from . import _lib
+    colptr = np.ascontiguousarray(colptr, dtype=np.int32)
+    rowidx = np.ascontiguousarray(rowidx, dtype=np.int32)
+    perm = np.zeros(int(n), dtype=np.int64)
+    _lib.check(_lib.lib().swc_nd_order(int(n), ctypes.c_void_p(colptr.ctypes.data),
+                                       ctypes.c_void_p(rowidx.ctypes.data),
+                                       ctypes.c_void_p(perm.ctypes.data)))
+    return perm
+
+
+def mesh2d_random(npts=20000, seed=5, wlo=1e-1, whi=1e1, shift=1e-4):
+    """Variable-coefficient Laplacian on an unstructured 2D mesh: the Delaunay
+    triangulation of `npts` uniform random points in the unit square, edge
+    weights log-uniform in [wlo, whi], diagonal = sum of incident weights +
+    shift; ordered by nd_order (no grid to cut along)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0, 1, (npts, 2))
+    tri = Delaunay(pts).simplices
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [0, 2]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    w = _loguniform(rng, wlo, whi, len(e))
+    diag = np.full(npts, shift)
+    np.add.at(diag, e[:, 0], w)
+    np.add.at(diag, e[:, 1], w)
+    rows = np.concatenate([e[:, 1], np.arange(npts)])
+    cols = np.concatenate([e[:, 0], np.arange(npts)])
+    vals = np.concatenate([-w, diag])
+    colptr, rowidx, values = lower_csc_from_coo(npts, rows, cols, vals)
+    perm = nd_order(npts, colptr, rowidx)
+    return Problem(npts, colptr, rowidx, values, perm, f"mesh2d_{npts}")
+
+
 def write_mtx(path, n, colptr, rowidx, values):
     """MatrixMarket lower triangle sorted by (col,row), as ``readMatrix`` expects."""
     cols = np.repeat(np.arange(n), np.diff(colptr))

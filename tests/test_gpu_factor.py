@@ -304,3 +304,19 @@ def test_factor_from_mtx_and_perm_files_exported_as_csc(tmp_path):
     Cp, Ci, Cx = api.bcsc2csc(b, Lo)
     assert np.array_equal(Lc.indptr, Cp) and np.array_equal(Lc.indices, Ci)
     assert np.linalg.norm(Lc.data - Cx) <= 1e-12 * np.linalg.norm(Cx)
+
+
+def test_unstructured_mesh_with_own_nd_order():
+    """An operator without grid coordinates (Delaunay mesh of 20,000 random
+    points) ordered by the library's nested dissection: GPU factor within 1e-12
+    of the oracle's, solve to backward error 1e-14."""
+    prob = P.mesh2d_random(20000)
+    b, a2x, Lo, info = setup(prob)
+    assert info == 0
+    f = S.Factor(b)
+    assert f.factor(prob.values) == 0
+    assert relF(f.L_values(), Lo) <= 1e-12
+    A, rp, ci, v = full_csr(prob)
+    rhs = A @ np.random.default_rng(9).uniform(-1, 1, prob.n)
+    x = f.solve(rhs)
+    assert O.backward_error(rp, ci, v, x, rhs) <= 1e-14

@@ -98,3 +98,35 @@ def test_problem_round_trips_through_mtx_and_perm_files(tmp_path):
     b1 = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     b2 = S.analyze_p2(n, cp, ri, perm)
     assert np.array_equal(b1.s, b2.s) and np.array_equal(b1.pruneSet, b2.pruneSet)
+
+
+def test_nd_order_of_general_patterns():
+    """swc_nd_order (the METIS_NodeND stand-in, LSparsity.h:597): a bijection, and
+    a fill-reducing one -- on grids no more fill than the geometric
+    nested dissection, on an unstructured Delaunay mesh a fraction of reverse
+    Cuthill-McKee's fill and of the natural order's."""
+    import numpy as np
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    import swcholesky_amd as S
+
+    def nnz_l(prob, perm):
+        b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, perm)
+        return int(np.sum(b.ColCount))
+
+    for prob in (P.laplacian2d_var(64, 3), P.laplacian3d_var(16), P.vector3d_var(6)):
+        perm = P.nd_order(prob.n, prob.colptr, prob.rowidx)
+        assert np.array_equal(np.sort(perm), np.arange(prob.n))
+        assert nnz_l(prob, perm) <= 1.05 * nnz_l(prob, prob.perm)
+    m = P.mesh2d_random(6000)
+    assert np.array_equal(np.sort(m.perm), np.arange(m.n))
+    rcm = reverse_cuthill_mckee(m.scipy_full(), symmetric_mode=True).astype(np.int64)
+    assert nnz_l(m, m.perm) < 0.5 * nnz_l(m, rcm)
+    assert nnz_l(m, m.perm) < 0.1 * nnz_l(m, np.arange(m.n, dtype=np.int64))
+    # a pattern with several components and an isolated vertex
+    import scipy.sparse as sp
+    A = sp.block_diag([P.laplacian2d_var(10, 1).scipy_full(), sp.identity(1),
+                       P.laplacian2d_var(7, 2).scipy_full()]).tocsc()
+    low = sp.tril(A).tocsc()
+    low.sort_indices()
+    perm = P.nd_order(A.shape[0], low.indptr, low.indices)
+    assert np.array_equal(np.sort(perm), np.arange(A.shape[0]))
