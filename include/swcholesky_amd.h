@@ -137,12 +137,13 @@ int swc_upload_symbolic(swc_handle* h, const swc_symbolic* sym);
 int swc_set_perm(swc_handle* h, const int* perm);
 
 /* Shape parameters of the plan built by the next structure upload (swc_upload,
- * swc_upload_symbolic), as {rlb, spw, narrow_min_tiles, narrow_max_k} (count <= 4,
- * leading entries; NULL restores the defaults {8, 128, 1024, 256}): rlb panels per
- * block of a wide supernode's right-looking steps, spw = 64 or 128 columns per
- * solve panel, and the one-wave tile kernel for levels with at least
+ * swc_upload_symbolic), as {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow}
+ * (count <= 5, leading entries; NULL restores the defaults {8, 128, 1024, 256, 1}):
+ * rlb panels per block of a wide supernode's right-looking steps, spw = 64 or 128
+ * columns per solve panel, the one-wave tile kernel for levels with at least
  * narrow_min_tiles tiles of panels <= 16 columns whose inner dimensions sum to
- * <= narrow_max_k.  No reference counterpart (the reference's Sunway BLAS fixes
+ * <= narrow_max_k, and with fuse_narrow = 1 such panels (all tiles narrow) updated
+ * and factored by one wave each in one persistent launch per level.  No reference counterpart (the reference's Sunway BLAS fixes
  * its blocking, BLAS-sw/dpotrf_cluster_master.c:11-63); the values come from the
  * autotuner's table (swcholesky_amd/tuning.json). */
 int swc_set_plan_params(swc_handle* h, const int* params, int count);
@@ -249,7 +250,9 @@ int swc_get_timings(const swc_handle* h, double* out);
  * vsize), 9 factor-list tiles, 10 solve row blocks, 11 solve level->row-block
  * offsets, 12 gathered columns by solve level, 13 solve level->gathered-column
  * offsets, 14 solve panels (Panel structs), 15 solve panels by level, 16 solve
- * level->solve-panel offsets) as the packed structs and arrays of
+ * level->solve-panel offsets, 17 fused narrow panels (int pairs: first tile, tile
+ * count), 18 level->fused-panel offsets, 19 per level: tiles of fused panels, 20
+ * per level: one-wave tiles) as the packed structs and arrays of
  * swcholesky_amd/csrc/plan.h.  With prunePtr NULL the descendant sets are derived
  * from the row lists (supernode s depends on every d whose rows below its own
  * columns meet s's columns) and only the solve part of the plan is built, as

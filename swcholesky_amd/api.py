@@ -171,7 +171,7 @@ class Factor:
     def __init__(self, bcsc: BCSC, params="auto", schedule=None):
         """params: "auto" (the autotuner's table entry for this structure's class,
         swcholesky_amd/tuning.py), None (the library defaults) or a sequence
-        {rlb, spw, narrow_min_tiles, narrow_max_k} (swc_set_plan_params).
+        {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow} (swc_set_plan_params).
         schedule: None (wavefront levels) or the reference's H-level arrays
         (levelPtr, parPtr, partition) (swc_set_schedule)."""
         self.bcsc = bcsc
@@ -379,20 +379,23 @@ def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
                                     0 if lp is None else len(lp) - 1, ptr(lp), ptr(pr), ptr(part),
                                     ctypes.byref(h)))
     try:
-        sz = np.zeros(12, dtype=np.int64)
+        sz = np.zeros(13, dtype=np.int64)
         fl = (ctypes.c_double * 2)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, nsp, dscr, nslevel = (
+        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, nsp, dscr, nslevel, nfused = (
             int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
                 (6, np.int32, nlevel + 1), (7, np.int64, n + 1), (8, np.int64, vsize),
                 (9, np.int32, ntrsm), (10, SBLK_DT, nfblk), (11, np.int32, nslevel + 1),
                 (12, np.int32, n), (13, np.int32, nslevel + 1), (14, PANEL_DT, nsp),
-                (15, np.int32, nsp), (16, np.int32, nslevel + 1)]
+                (15, np.int32, nsp), (16, np.int32, nslevel + 1),
+                (17, np.dtype([("tile0", np.int32), ("ntile", np.int32)]), nfused),
+                (18, np.int32, nlevel + 1), (19, np.int32, nlevel), (20, np.int32, nlevel)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
                  "lvl_panels", "gptr", "gidx", "trsm_tiles", "fblocks", "lvl_fblk",
-                 "gcols", "lvl_gcol", "spanels", "sp_order", "lvl_sp"]
+                 "gcols", "lvl_gcol", "spanels", "sp_order", "lvl_sp", "fused", "lvl_fused",
+                 "lvl_tiles_fused", "lvl_tiles_narrow"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)

@@ -412,6 +412,141 @@ k_panel_factor_regs(double* __restrict__ L, const Tile* __restrict__ ftiles, int
 }
 
 
+// Fused narrow panels of a level: one persistent launch whose waves take chunks
+// of consecutive panels from a device worklist (a counter claimed with a vector
+// atomic, the next claim issued while the current chunk is worked on) until the
+// level's list is exhausted.  A panel (<= 16 columns, a single-panel supernode whose tiles all
+// have short updates) is done by one wave from start to finish: every tile's
+// updates as k_tile_narrow applies them, the updated tile handed from the MFMA
+// layout to row-per-lane registers through the wave's own LDS buffer, the
+// diagonal block factored and the rows solved as k_panel_factor_regs does it
+// (the same operations in the same order, so the result does not depend on
+// which wave takes which panel), no copy of the diagonal block through global
+// memory and no second launch.  MODE as k_tile_update; Lf: the FP32 copy of L,
+// read by FP32 updates and written with the factor when write_f32 is set.
+template <int MODE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_narrow_fused(double* __restrict__ L, const Tile* __restrict__ tiles,
+               const Task* __restrict__ tasks, const swc::FusedPanel* __restrict__ items,
+               int nitems, int chunk, int* __restrict__ counter, int* __restrict__ info_out,
+               float* Lf, int write_f32) {
+  constexpr int NC = 16;
+  __shared__ double T[swc::TM * (NC + 1)];  // updated tile, row i at T[i * (NC + 1)]
+  __shared__ double L11[NC * (NC + 1)];     // the panel's factored diagonal block, row-major
+  const int lane = threadIdx.x;
+  // the worklist hands out chunks of `chunk` consecutive panels
+  int claim = 0;
+  if (lane == 0) claim = atomicAdd(counter, 1);
+  int item = __builtin_amdgcn_readfirstlane(claim) * chunk;
+  int end = min(item + chunk, nitems);
+  if (lane == 0) claim = atomicAdd(counter, 1);  // the next chunk, used after this one
+  while (item < nitems) {
+    if (item == end) {
+      item = __builtin_amdgcn_readfirstlane(claim) * chunk;
+      if (item >= nitems) break;
+      end = min(item + chunk, nitems);
+      if (lane == 0) claim = atomicAdd(counter, 1);
+    }
+    const swc::FusedPanel it = items[item];
+    for (int t = 0; t < it.ntile; ++t) {
+      const Tile tl = tiles[it.tile0 + t];
+      const int nr = tl.nr, nc = tl.ncol, ld = tl.ld;
+      const bool diag = tl.r0 == 0;
+      double* gtile = L + tl.vbase + tl.r0;
+      // (1) the tile's updates (k_tile_narrow)
+      d4 acc[4][1];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) acc[bi][0] = d4{0.0, 0.0, 0.0, 0.0};
+      const bool f32 = MODE == 2 || (MODE == 1 && tl.task_end > tl.task_begin &&
+                                       tasks[tl.task_begin].kind == 0);
+      if (f32) {
+        for (int q = tl.task_begin; q < tl.task_end; ++q) {
+          const Task tk = tasks[q];
+          task_update<float, 1, 16>(Lf + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask,
+                                    tk.cmask, diag, 0, lane, acc);
+        }
+      } else {
+        for (int q = tl.task_begin; q < tl.task_end; ++q) {
+          const Task tk = tasks[q];
+          task_update<double, 1, 16>(L + tk.src_base, tk.ld, tk.k, tk.ra, tk.ca, tk.rmask,
+                                     tk.cmask, diag, 0, lane, acc);
+        }
+      }
+      {
+        const int j = lane & 15;
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 16 * bi + (lane >> 4) + 4 * r;
+            const bool in = i < nr && j < nc && !(diag && i < j);
+            const double g = gtile[in ? (int64_t)j * ld + i : 0];
+            T[i * (NC + 1) + j] = in ? g - acc[bi][0][r] : 0.0;
+          }
+      }
+      float* ftile = write_f32 ? Lf + tl.vbase + tl.r0 : nullptr;
+      if (t == 0) {
+        // (2) POTRF of the diagonal block (k_panel_factor_regs), kept in LDS
+        double y[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] = (lane < nc && c < nc && c <= lane) ? T[lane * (NC + 1) + c] : 0.0;
+        int badcol = 0;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          if (j < nc) {
+            const double d = bcast(y[j], j);
+            const bool ok = d > 0.0;
+            if (!ok && badcol == 0) badcol = j + 1;
+            const double rs = ok ? rsqrt(d) : 1.0;
+            if (lane == j) y[j] = ok ? d * rs : 1.0;
+            else if (lane > j) y[j] *= rs;
+#pragma unroll
+            for (int c = j + 1; c < NC; ++c)
+              if (c < nc) y[c] -= y[j] * bcast(y[j], c);
+          }
+        }
+        if (badcol && lane == 0) atomicMin(info_out, tl.c0 + badcol);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          if (lane < nc && c <= lane) {
+            gtile[(int64_t)c * ld + lane] = y[c];
+            if (ftile) ftile[(int64_t)c * ld + lane] = (float)y[c];
+          }
+          if (lane < NC) L11[lane * (NC + 1) + c] = y[c];
+        }
+      }
+      // (3) the tile's rows below the diagonal block: x <- x L11^{-T}
+      const int first = t == 0 ? nc : 0;
+      const bool mine = lane >= first && lane < nr;
+      double x[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) x[c] = (mine && c < nc) ? T[lane * (NC + 1) + c] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        if (j < nc) {
+          x[j] *= 1.0 / L11[j * (NC + 1) + j];
+#pragma unroll
+          for (int c = j + 1; c < NC; ++c)
+            if (c < nc) x[c] -= x[j] * L11[c * (NC + 1) + j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (mine && c < nc) {
+          gtile[(int64_t)c * ld + lane] = x[c];
+          if (ftile) ftile[(int64_t)c * ld + lane] = (float)x[c];
+        }
+    }
+    ++item;
+  }
+}
+
+__global__ void k_fill_int(int* __restrict__ x, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
 // Column stride (doubles) of the LDS working matrix of k_panel_factor_blk: rows
 // 0..63 hold the panel's diagonal block, rows 64..127 the tile rows being solved.
 // An odd stride spreads the 16 columns an MFMA output block touches over all
@@ -1610,6 +1745,9 @@ struct swc_handle {
   double* dY = nullptr;   // forward solutions, n * work_cols
   int* dOrder = nullptr;
   Tile* dFtiles = nullptr;  // factor-kernel tiles (copies), by level and width class
+  swc::FusedPanel* dFused = nullptr;  // fused narrow panels, by level
+  int* dCounters = nullptr;  // per level: the fused kernel's worklist counter
+  int fused_cap[3] = {0, 0, 0};  // per precision mode: fused-kernel workgroups resident at once
   double* dDiag = nullptr;  // per-level copies of the diagonal blocks being factored
   int64_t* dAmap = nullptr;
   int* dPerm = nullptr;
@@ -1683,14 +1821,14 @@ struct swc_handle {
   void release() {
     drop_graphs();
     (void)hipFree(dL); (void)hipFree(dLf); (void)hipFree(dLs); (void)hipFree(dPanels); (void)hipFree(dTiles); (void)hipFree(dTasks);
-    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dFtiles); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
+    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dFtiles); (void)hipFree(dFused); (void)hipFree(dCounters); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
     (void)hipFree(dVals); (void)hipFree(dWork); (void)hipFree(dInfo);
     (void)hipFree(dRp); (void)hipFree(dCi); (void)hipFree(dVmap); (void)hipFree(dAf); (void)hipFree(dNorm); (void)hipFree(dScal); (void)hipFree(dPart); (void)hipFree(dNormPart);
     (void)hipFree(dRef);
     (void)hipFree(dZ); (void)hipFree(dArena); (void)hipFree(dNodes); (void)hipFree(dCol2sup); (void)hipFree(dSpx);
     (void)hipFree(dSsuper); (void)hipFree(dSpi); (void)hipFree(dDiagPos); (void)hipFree(dQi); (void)hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dFtiles = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dFtiles = nullptr; dFused = nullptr; dCounters = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dNorm = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     lf_cap = 0;
@@ -1793,23 +1931,39 @@ void enqueue_factor(swc_handle* h, bool profile) {
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, (const double*)h->dNormPart, 0,
                      h->dNorm);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, 0x7fffffff);
+  hipLaunchKernelGGL(k_fill_int, dim3((std::max(P.nlevel, 1) + kThreads - 1) / kThreads),
+                     dim3(kThreads), 0, st, h->dCounters, std::max(P.nlevel, 1), 0);
   for (int l = 0; l < P.nlevel; ++l) {
     const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0;
     const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
     hipEvent_t* pe = profile ? &h->pev[(size_t)4 * l] : nullptr;
     if (pe) (void)hipEventRecord(pe[0], st);
-    const int ntn = P.lvl_tiles_narrow[l];  // the level's one-wave tiles come first
+    // the level's tiles: fused narrow panels first, then one-wave tiles, then the rest
+    const int ntf = P.lvl_tiles_fused[l], ntn = P.lvl_tiles_narrow[l];
+    const int f0 = P.lvl_fused[l], nf = P.lvl_fused[l + 1] - f0;
+    if (nf > 0) {
+      auto kern = h->precision == 0 ? k_narrow_fused<0> : (h->precision == 1 ? k_narrow_fused<1>
+                                                                              : k_narrow_fused<2>);
+      // about four chunks per workgroup: few claims on the one counter, and the
+      // last chunks still balance the waves' loads
+      const int grid = std::min(nf, h->fused_cap[h->precision]);
+      const int chunk = std::max(1, (nf + 4 * grid - 1) / (4 * grid));
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, st, h->dL, (const Tile*)h->dTiles,
+                         (const Task*)h->dTasks, (const swc::FusedPanel*)h->dFused + f0, nf, chunk,
+                         h->dCounters + l, h->dInfo, h->dLf, h->fp32_mask ? 1 : 0);
+    }
     if (ntn > 0) {
       auto kern = h->precision == 0 ? k_tile_narrow<0> : (h->precision == 1 ? k_tile_narrow<1>
                                                                              : k_tile_narrow<2>);
       hipLaunchKernelGGL(kern, dim3(ntn), dim3(64), 0, st, h->dL, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0, h->dDiag, (const float*)h->dLf);
+                         (const Task*)h->dTasks, t0 + ntf, h->dDiag, (const float*)h->dLf);
     }
-    if (nt > ntn) {
+    if (nt > ntf + ntn) {
       auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
                                                                              : k_tile_update<2>);
-      hipLaunchKernelGGL(kern, dim3(nt - ntn), dim3(kThreads), 0, st, h->dL, (const Tile*)h->dTiles,
-                         (const Task*)h->dTasks, t0 + ntn, h->dDiag, (const float*)h->dLf);
+      hipLaunchKernelGGL(kern, dim3(nt - ntf - ntn), dim3(kThreads), 0, st, h->dL,
+                         (const Tile*)h->dTiles, (const Task*)h->dTasks, t0 + ntf + ntn, h->dDiag,
+                         (const float*)h->dLf);
     }
     if (pe) (void)hipEventRecord(pe[1], st);
     if (ns > 0) {
@@ -2373,6 +2527,20 @@ int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const
   if (!rc) rc = upload(&h->dSpOrder, P.sp_order.data(), P.sp_order.size(), st);
   if (!rc) rc = upload(&h->dOrder, P.panel_order.data(), P.panel_order.size(), st);
   if (!rc) rc = upload(&h->dFtiles, P.ftiles.data(), P.ftiles.size(), st);
+  if (!rc) rc = upload(&h->dFused, P.fused.data(), P.fused.size(), st);
+  if (!rc) rc = upload(&h->dCounters, (const int*)nullptr, (size_t)std::max(P.nlevel, 1), st);
+  if (!rc && h->fused_cap[0] == 0) {
+    // a persistent grid: as many workgroups as the device holds at once
+    int ncu = 0, per[3] = {0, 0, 0};
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_narrow_fused<0>, 64, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_narrow_fused<1>, 64, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_narrow_fused<2>, 64, 0);
+    if (e != hipSuccess)
+      rc = swc_set_error(SWC_EDEVICE, std::string("upload: occupancy query: ") + hipGetErrorString(e));
+    else
+      for (int m = 0; m < 3; ++m) h->fused_cap[m] = std::max(1, per[m]) * std::max(ncu, 1);
+  }
   if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
   if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
   if (!rc) rc = upload(&h->dNormPart, (const double*)nullptr, kRedBlocks, st);

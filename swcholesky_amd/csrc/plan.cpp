@@ -62,8 +62,10 @@ bool plan_params_from(const int* v, int count, PlanParams& out) {
   if (count > 1) p.spw = v[1];
   if (count > 2) p.narrow_min_tiles = v[2];
   if (count > 3) p.narrow_max_k = v[3];
+  if (count > 4) p.fuse_narrow = v[4];
   if (count < 0 || count > kPlanParams || p.rlb < 1 || p.rlb > 64 ||
-      (p.spw != 64 && p.spw != SPW) || p.narrow_min_tiles < 1 || p.narrow_max_k < 0)
+      (p.spw != 64 && p.spw != SPW) || p.narrow_min_tiles < 1 || p.narrow_max_k < 0 ||
+      (p.fuse_narrow != 0 && p.fuse_narrow != 1))
     return false;
   out = p;
   return true;
@@ -435,15 +437,40 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     for (size_t i = 0; i < ptiles.size(); ++i)
       if (cnt[ptiles[i].level] < params.narrow_min_tiles) is_narrow[i] = 0;
   }
+  // Panels (<= 16 columns, so single-panel supernodes, factored at the level of
+  // their updates) whose tiles are all narrow are fused: one wave updates and
+  // factors the whole panel, and they leave the narrow and factor launches.
+  std::vector<char> fused_panel(P.npanel, 0);
+  if (params.fuse_narrow) {
+    std::vector<char> all_narrow(P.npanel, 1), has_diag(P.npanel, 0);
+    for (size_t i = 0; i < ptiles.size(); ++i) {
+      const int q = ptiles[i].t.panel;
+      if (!is_narrow[i]) all_narrow[q] = 0;
+      if (ptiles[i].t.kind == 0) has_diag[q] = 1;
+    }
+    for (int q = 0; q < P.npanel; ++q)
+      fused_panel[q] = all_narrow[q] && has_diag[q] && P.panels[q].ncol <= 16 &&
+                       P.panels[q].sw == P.panels[q].ncol;
+  }
+  auto fused = [&](int i) { return fused_panel[ptiles[i].t.panel] != 0; };
   auto narrow = [&](int i) { return is_narrow[i] != 0; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (ptiles[a].level != ptiles[b].level) return ptiles[a].level < ptiles[b].level;
+    if (fused(a) != fused(b)) return fused(a);
+    if (fused(a)) {  // grouped by panel, the diagonal tile first
+      if (ptiles[a].t.panel != ptiles[b].t.panel) return ptiles[a].t.panel < ptiles[b].t.panel;
+      return ptiles[a].t.r0 < ptiles[b].t.r0;
+    }
     if (narrow(a) != narrow(b)) return narrow(a);
     return cost[a] > cost[b];
   });
+  P.lvl_tiles_fused.assign(P.nlevel, 0);
   P.lvl_tiles_narrow.assign(P.nlevel, 0);
-  for (size_t i = 0; i < ptiles.size(); ++i)
-    if (is_narrow[i]) P.lvl_tiles_narrow[ptiles[i].level]++;
+  for (size_t i = 0; i < ptiles.size(); ++i) {
+    if (fused((int)i)) P.lvl_tiles_fused[ptiles[i].level]++;
+    else if (is_narrow[i]) P.lvl_tiles_narrow[ptiles[i].level]++;
+  }
+  P.lvl_fused.assign(P.nlevel + 1, 0);
   P.lvl_tiles.assign(P.nlevel + 1, 0);
   P.lvl_trsm.assign(P.nlevel + 1, 0);
   P.lvl_tile_flops.assign(P.nlevel, 0.0);
@@ -482,9 +509,18 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     }
     pt.t.task_end = (int)P.tasks.size();
     P.lvl_tiles[pt.level + 1]++;
-    if (pt.trsm || pt.t.kind == 0) trsm_by_level[pt.level].push_back((int)P.tiles.size());
+    if (fused(oi)) {
+      if (pt.t.kind == 0) {
+        P.fused.push_back({(int)P.tiles.size(), 0});
+        P.lvl_fused[pt.level + 1]++;
+      }
+      P.fused.back().ntile++;
+    } else if (pt.trsm || pt.t.kind == 0) {
+      trsm_by_level[pt.level].push_back((int)P.tiles.size());
+    }
     P.tiles.push_back(pt.t);
   }
+  for (int l = 0; l < P.nlevel; ++l) P.lvl_fused[l + 1] += P.lvl_fused[l];
   for (int l = 0; l < P.nlevel; ++l) P.lvl_tiles[l + 1] += P.lvl_tiles[l];
   // scratch slots for the diagonal blocks each level factors: the update kernel
   // copies the diagonal tile's block there, every factor-kernel tile reads it
@@ -631,6 +667,7 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[9] = (int64_t)P.spanels.size();
   out[10] = P.diag_scratch;
   out[11] = P.nslevel;
+  out[12] = (int64_t)P.fused.size();
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
@@ -656,6 +693,10 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 7: return cp(P.gptr);
     case 8: return cp(P.gidx);
     case 9: return cp(P.trsm_tiles);
+    case 17: return cp(P.fused);
+    case 18: return cp(P.lvl_fused);
+    case 19: return cp(P.lvl_tiles_fused);
+    case 20: return cp(P.lvl_tiles_narrow);
     case 10: return cp(P.fblocks);
     case 11: return cp(P.lvl_fblk);
     case 12: return cp(P.gcols);

@@ -31,8 +31,11 @@ struct PlanParams {
   int narrow_min_tiles = 1024; // a level runs its narrow tiles one wave each (own launch)
   int narrow_max_k = 256;      // ... if it has this many; narrow = panel <= 16 columns and
                                // total inner dimension <= narrow_max_k
+  int fuse_narrow = 1;         // 1: a level's panels whose tiles are all narrow are updated and
+                               // factored together, one wave per panel from a device worklist
 };
-constexpr int kPlanParams = 4;  // entries of the int[] form {rlb, spw, narrow_min_tiles, narrow_max_k}
+// entries of the int[] form {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow}
+constexpr int kPlanParams = 5;
 // Checks and copies the int[] form; returns false if a value is out of range.
 bool plan_params_from(const int* v, int count, PlanParams& out);
 
@@ -91,6 +94,13 @@ struct SolveBlock {
   int writer;
 };
 
+// A panel updated and factored by one wave of the fused narrow-panel kernel:
+// its tiles are tiles[tile0 .. tile0 + ntile), the diagonal tile first.
+struct FusedPanel {
+  int tile0;
+  int ntile;
+};
+
 static_assert(sizeof(Panel) == 48, "Panel layout is part of the plan export");
 static_assert(sizeof(Tile) == 48, "Tile layout is part of the plan export");
 static_assert(sizeof(Task) == 56, "Task layout is part of the plan export");
@@ -102,8 +112,11 @@ struct Plan {
   std::vector<Panel> panels;
   std::vector<Tile> tiles;          // grouped by level (one update launch per level)
   std::vector<int> lvl_tiles;       // nlevel+1 offsets into tiles
+  std::vector<int> lvl_tiles_fused;   // nlevel: tiles of fused narrow panels, first in their level
   std::vector<int> lvl_tiles_narrow;  // nlevel: one-wave tiles (panels <= 16 wide, short
-                                      // total inner dimension), first in their level
+                                      // total inner dimension), next in their level
+  std::vector<FusedPanel> fused;      // fused narrow panels, by level
+  std::vector<int> lvl_fused;         // nlevel+1 offsets into fused
   std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)
   std::vector<Tile> ftiles;         // copies of those tiles in the same order
   std::vector<int> lvl_trsm;        // nlevel+1 offsets into trsm_tiles

@@ -64,6 +64,29 @@ def factor(plan, Ls, xsize, amap, vals):
             if tl["slot"] >= 0 and tl["kind"] == 0:
                 _chk(int(tl["slot"]), int(tl["slot"]) + nc * nc, plan["diag_scratch"], "diag slot")
                 scratch[int(tl["slot"])] = np.tril(acc[:nc, :nc]).copy()
+        # fused narrow panels: one wave updates (above) and factors each panel,
+        # its tiles contiguous with the diagonal tile first, none in the factor lists
+        if "fused" in plan:
+            in_lists = set(int(t) for t in plan["trsm_tiles"][s0:s1])
+            for it in plan["fused"][plan["lvl_fused"][lvl]:plan["lvl_fused"][lvl + 1]]:
+                t0f, ntf = int(it["tile0"]), int(it["ntile"])
+                assert t0 <= t0f and t0f + ntf <= t0 + plan["lvl_tiles_fused"][lvl]
+                d = tiles[t0f]
+                pn = panels[d["panel"]]
+                nc, ld = int(pn["ncol"]), int(pn["ld"])
+                assert d["kind"] == 0 and d["r0"] == 0 and nc <= 16 and int(pn["sw"]) == nc
+                didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
+                a11 = np.tril(L[didx])
+                for k in range(ntf):
+                    tl = tiles[t0f + k]
+                    assert tl["panel"] == d["panel"] and t0f + k not in in_lists
+                    assert k == 0 or tl["r0"] > tiles[t0f + k - 1]["r0"]
+                    first = nc if k == 0 else 0
+                    base = int(pn["vbase"]) + int(tl["r0"])
+                    assert (np.arange(64)[:, None] * 17 + np.arange(16)[None, :]).max() < 64 * 17
+                    bad = _panel_factor_scalar(L, base, ld, a11.copy(), nc, int(tl["nr"]), first, xsize)
+                    if bad and k == 0:
+                        info = info or int(pn["c0"]) + bad
         # panel factor kernel: every listed tile recomputes L11 of its panel
         for ti in plan["trsm_tiles"][s0:s1]:
             tl = tiles[ti]

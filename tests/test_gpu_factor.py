@@ -320,3 +320,24 @@ def test_unstructured_mesh_with_own_nd_order():
     rhs = A @ np.random.default_rng(9).uniform(-1, 1, prob.n)
     x = f.solve(rhs)
     assert O.backward_error(rp, ci, v, x, rhs) <= 1e-14
+
+
+@pytest.mark.parametrize("name,make", [SMALL[0], SMALL[3], SMALL[5]],
+                         ids=[SMALL[0][0], SMALL[3][0], SMALL[5][0]])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_narrow_panels_match_two_kernel_path(name, make, mode):
+    """Narrow panels updated and factored by the persistent worklist kernel (one
+    wave per panel) give bitwise the factor of the update + factor launches they
+    replace, and the oracle's within 1e-12 (FP64); forced on small operators with
+    a one-tile threshold."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    out = {}
+    for fuse in (1, 0):
+        f = S.Factor(b, params=(8, 128, 1, 256, fuse))
+        f.set_precision(mode)
+        assert f.factor(prob.values) == 0
+        out[fuse] = f.L_values()
+    assert np.array_equal(out[0], out[1])
+    if mode == 0:
+        assert relF(out[1], Lo) <= 1e-12

@@ -149,8 +149,10 @@ def test_derived_descendant_sets_give_the_reference_solve_schedule(name):
 
 
 @pytest.mark.parametrize("params", [(2, 128, 1024, 256), (16, 64, 1024, 256), (8, 64, 1, 512),
-                                    (1, 128, 1, 0), (64, 64, 4096, 128)])
-@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(12), lambda: P.dense_spd(300)])
+                                    (1, 128, 1, 0), (64, 64, 4096, 128), (8, 128, 1, 256, 1),
+                                    (8, 128, 1, 512, 0)])
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(12), lambda: P.dense_spd(300),
+                                  lambda: P.laplacian2d_var(37, 3)])
 def test_plan_shape_parameters_keep_parity(params, make):
     """Every plan shape the autotuner may pick (right-looking block size, solve
     panel width, one-wave tile thresholds) factors and solves to the oracle's
@@ -178,7 +180,7 @@ def test_plan_shape_parameters_are_checked_and_looked_up():
             api.build_plan(b, params=bad)
     entries = [{"class": {"n": 262144, "max_width": 6593}, "params": dict(zip(tuning.NAMES, (4, 64, 256, 128)))},
                {"class": {"n": 1369, "max_width": 40}, "params": dict(zip(tuning.NAMES, (16, 128, 4096, 512)))}]
-    assert tuning.params_for(b, entries) == (16, 128, 4096, 512)
+    assert tuning.params_for(b, entries) == (16, 128, 4096, 512, 1)  # missing keys: defaults
     assert tuning.params_for(b, []) == tuning.DEFAULTS
 
 
