@@ -189,6 +189,19 @@ def main():
                    "iters": iters, "info": info}
             if best is None or dt < best["ms"]:
                 best = rec
+        # the parts: one solve with its residual evaluation (no correction), and
+        # the cost of each correction (a residual, a solve and the reductions)
+        x0 = drhs.clone()
+        f.refine(x0, tol=0.0, maxiter=0)  # captures the solve sequence for this buffer
+        x0.copy_(drhs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f.refine(x0, tol=0.0, maxiter=0)
+        torch.cuda.synchronize()
+        best["solve_and_residual_ms"] = (time.perf_counter() - t0) * 1e3
+        refine_ms = best["ms"] - best["factor_ms_device"]
+        best["per_correction_ms"] = ((refine_ms - best["solve_and_residual_ms"]) / best["iters"]
+                                     if best["iters"] else 0.0)
         xh = x.cpu().numpy()
         r = rhs - A @ xh
         best["eta_host_check"] = float(np.abs(r).max() / (abs(A).sum(axis=1).max() * np.abs(xh).max()
@@ -196,7 +209,9 @@ def main():
         best["factor_gflops"] = b.flops / (best["factor_ms_device"] * 1e-3) / 1e9
         paths[name] = best
         log(f"{name}: time-to-1e-12 {best['ms']:.1f} ms (factor {best['factor_ms_device']:.1f} ms, "
-            f"{best['iters']} corrections, eta {best['eta']:.2e} / host {best['eta_host_check']:.2e})")
+            f"solve + residual {best['solve_and_residual_ms']:.1f} ms, {best['iters']} corrections "
+            f"of {best['per_correction_ms']:.1f} ms, eta {best['eta']:.2e} / host "
+            f"{best['eta_host_check']:.2e})")
     f.set_precision(0)
 
     # roofline of the dominant kernel from per-launch events
