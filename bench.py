@@ -111,7 +111,12 @@ def main():
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device-to-device copy of this many bytes (a plain "
                          "copy kernel of known size, to scale HBM counters)")
+    ap.add_argument("--params", default=None,
+                    help="plan shape parameters as comma-separated integers {rlb, spw, "
+                         "narrow_min_tiles, narrow_max_k, fuse_narrow, dag} (default: the "
+                         "tuning table's entry for the structure)")
     args = ap.parse_args()
+    args.plan = tuple(int(v) for v in args.params.split(",")) if args.params else "auto"
     if args.gpus != 1:
         print("bench.py measures one GPU: the library factors each matrix on one GPU "
               "and has no multi-GPU path", file=sys.stderr)
@@ -132,7 +137,7 @@ def main():
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     t_an = time.perf_counter() - t0
     log(f"analysis {t_an:.2f}s nsuper={b.nsuper} xsize={b.xsize} F={b.flops:.4e}")
-    f = S.Factor(b)
+    f = S.Factor(b, params=args.plan)
     stats = f.plan_stats()
     log(f"plan {stats}")
     dvals = torch.from_numpy(prob.values).cuda()
@@ -267,8 +272,8 @@ def main():
                                + ("geometric grid ND ordering" if args.ordering == "grid"
                                   else "graph ND ordering (swc_nd_order)"),
                    "n": prob.n, "nnz_tril": prob.nnz, "nsuper": b.nsuper, "flops_F": b.flops,
-                   "plan_params": dict(zip(("rlb", "spw", "narrow_min_tiles", "narrow_max_k"),
-                                           f.params or ())),
+                   "plan_params": dict(zip(("rlb", "spw", "narrow_min_tiles", "narrow_max_k",
+                                            "fuse_narrow", "dag"), f.params or ())),
                    "parallelism": "single GPU"},
         "time_to_1e-12_ms": {k: v["ms"] for k, v in paths.items()},
         "mixed_speedup_vs_fp64": paths["fp64"]["ms"] / min(paths["mixed"]["ms"], paths["mixed_all"]["ms"]),
@@ -304,7 +309,7 @@ def bench_vec3d(args):
     prob = P.vector3d_var(side)
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
     log(f"vec3d {side}^3x3: n={prob.n} nsuper={b.nsuper} xsize={b.xsize} F={b.flops:.4e}")
-    f = S.Factor(b)
+    f = S.Factor(b, params=args.plan)
     dvals = torch.from_numpy(prob.values).cuda()
     rng = np.random.default_rng(7)
     res = {}
@@ -383,7 +388,7 @@ def bench_gmrf(args):
     side = 256 if args.side == 64 else args.side
     prob = P.gmrf2d(side)
     b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
-    f = S.Factor(b)
+    f = S.Factor(b, params=args.plan)
     vals = torch.tensor(prob.values, device="cuda", requires_grad=True)
     rhs = torch.tensor(np.random.default_rng(4).standard_normal(prob.n), device="cuda")
     for rep in range(2):

@@ -221,7 +221,10 @@ int swc_logdet(swc_handle* h, double* logdet);
  * with respect to the stored values when x = A^{-1} b (original ordering). */
 int swc_gradient(swc_handle* h, const double* x, int x_on_device, double alpha, double beta,
                  double* grad, int grad_on_device);
-/* out[5] = {panels, tiles, tasks, levels, xsize}; flops[2] = {update, factor}. */
+/* out[7] = {panels, tiles, tasks, levels, xsize, work items of the dependency-driven
+ * factor launch (0: none), its first level}; flops[5] = {update, factor, useful
+ * flops of the dependency-driven launch, its simulated makespan and that of the
+ * same levels as per-level launches (microseconds, the plan's cost model)}. */
 int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops);
 /* Per factor level l < cap: out[2l] = duration (ms) of its tile-update launch in
  * the last profiled factorization (swc_set_profiling), out[2l+1] = that
@@ -241,9 +244,13 @@ int swc_get_timings(const swc_handle* h, double* out);
 
 /* ---- planning introspection (host only, no reference counterpart) ----------
  * Builds the panel/tile/task plan the GPU kernels run, without touching the
- * device, for tools and tests.  sizes[12] = {panels, tiles, tasks, levels, n,
+ * device, for tools and tests.  sizes[16] = {panels, tiles, tasks, levels, n,
  * xsize, factor-list tiles, solve row blocks, contribution-vector size, solve
- * panels, diagonal-block scratch doubles, solve levels}; swc_plan_export copies
+ * panels, diagonal-block scratch doubles, solve levels, fused narrow panels,
+ * dependency-driven work items, first level of the dependency-driven launch,
+ * its dependency entries}; flops[5] = {update flops, factor flops, useful flops
+ * of the dependency-driven launch, its simulated makespan and that of the same
+ * levels as per-level launches (microseconds, cost model)}; swc_plan_export copies
  * array `which` (0 panels, 1 tiles, 2 tasks, 3 level->tile offsets, 4
  * level->factor-list offsets, 5 panels by level, 6 level->panel offsets, 7
  * column->gather offsets (int64, n+1), 8 gathered contribution offsets (int64,
@@ -252,7 +259,8 @@ int swc_get_timings(const swc_handle* h, double* out);
  * offsets, 14 solve panels (Panel structs), 15 solve panels by level, 16 solve
  * level->solve-panel offsets, 17 fused narrow panels (int pairs: first tile, tile
  * count), 18 level->fused-panel offsets, 19 per level: tiles of fused panels, 20
- * per level: one-wave tiles) as the packed structs and arrays of
+ * per level: one-wave tiles, 21 dependency-driven work items (DagItem), 22 their
+ * dependencies (int pairs: panel, tile count)) as the packed structs and arrays of
  * swcholesky_amd/csrc/plan.h.  With prunePtr NULL the descendant sets are derived
  * from the row lists (supernode s depends on every d whose rows below its own
  * columns meet s's columns) and only the solve part of the plan is built, as

@@ -329,12 +329,13 @@ class Factor:
         return out
 
     def plan_stats(self):
-        out = np.zeros(5, dtype=np.int64)
-        fl = (ctypes.c_double * 2)()
+        out = np.zeros(7, dtype=np.int64)
+        fl = (ctypes.c_double * 5)()
         check(_lib.lib().swc_plan_stats(self._h, ptr(out), fl))
         return dict(panels=int(out[0]), tiles=int(out[1]), tasks=int(out[2]),
-                    levels=int(out[3]), xsize=int(out[4]), update_flops=fl[0],
-                    factor_flops=fl[1])
+                    levels=int(out[3]), xsize=int(out[4]), dag_items=int(out[5]),
+                    dag_level0=int(out[6]), update_flops=fl[0], factor_flops=fl[1],
+                    dag_flops=fl[2], dag_sim_ms=fl[3] / 1e3, level_sim_ms=fl[4] / 1e3)
 
 
 PANEL_DT = np.dtype([("vbase", np.int64), ("voff", np.int64), ("ld", np.int32),
@@ -350,6 +351,8 @@ TASK_DT = np.dtype([("src_base", np.int64), ("ld", np.int32), ("k", np.int32),
                     ("rmask", np.uint64), ("cmask", np.uint64)])
 SBLK_DT = np.dtype([("panel", np.int32), ("r0", np.int32), ("nr", np.int32),
                     ("writer", np.int32)])
+DAG_DT = np.dtype([("tile", np.int32), ("fclass", np.int32), ("prev", np.int32),
+                   ("diag", np.int32), ("dep0", np.int32), ("dep1", np.int32)])
 
 
 def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
@@ -379,11 +382,11 @@ def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
                                     0 if lp is None else len(lp) - 1, ptr(lp), ptr(pr), ptr(part),
                                     ctypes.byref(h)))
     try:
-        sz = np.zeros(13, dtype=np.int64)
-        fl = (ctypes.c_double * 2)()
+        sz = np.zeros(16, dtype=np.int64)
+        fl = (ctypes.c_double * 5)()
         check(lib.swc_plan_sizes(h, ptr(sz), fl))
-        npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, nsp, dscr, nslevel, nfused = (
-            int(v) for v in sz)
+        (npanel, ntile, ntask, nlevel, n, xsize, ntrsm, nfblk, vsize, nsp, dscr, nslevel, nfused,
+         ndag, dag_level0, ndep) = (int(v) for v in sz)
         spec = [(0, PANEL_DT, npanel), (1, TILE_DT, ntile), (2, TASK_DT, ntask),
                 (3, np.int32, nlevel + 1), (4, np.int32, nlevel + 1), (5, np.int32, npanel),
                 (6, np.int32, nlevel + 1), (7, np.int64, n + 1), (8, np.int64, vsize),
@@ -391,18 +394,20 @@ def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
                 (12, np.int32, n), (13, np.int32, nslevel + 1), (14, PANEL_DT, nsp),
                 (15, np.int32, nsp), (16, np.int32, nslevel + 1),
                 (17, np.dtype([("tile0", np.int32), ("ntile", np.int32)]), nfused),
-                (18, np.int32, nlevel + 1), (19, np.int32, nlevel), (20, np.int32, nlevel)]
+                (18, np.int32, nlevel + 1), (19, np.int32, nlevel), (20, np.int32, nlevel),
+                (21, DAG_DT, ndag), (22, np.dtype([("panel", np.int32), ("count", np.int32)]), ndep)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
                  "lvl_panels", "gptr", "gidx", "trsm_tiles", "fblocks", "lvl_fblk",
                  "gcols", "lvl_gcol", "spanels", "sp_order", "lvl_sp", "fused", "lvl_fused",
-                 "lvl_tiles_fused", "lvl_tiles_narrow"]
+                 "lvl_tiles_fused", "lvl_tiles_narrow", "dag_items", "dag_deps"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)
             check(lib.swc_plan_export(h, which, ptr(a)))
             out[name] = a
         out.update(nlevel=nlevel, nslevel=nslevel, xsize=xsize, update_flops=fl[0],
-                   factor_flops=fl[1], vsize=vsize, diag_scratch=dscr)
+                   factor_flops=fl[1], vsize=vsize, diag_scratch=dscr, dag_level0=dag_level0,
+                   dag_flops=fl[2], dag_sim_us=fl[3], lvl_sim_us=fl[4])
         return out
     finally:
         lib.swc_plan_free(h)

@@ -61,6 +61,16 @@ __global__ void k_assemble(double* __restrict__ L, const int64_t* __restrict__ a
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
+// Workgroup barrier for LDS traffic only: it waits for this wave's LDS
+// operations but not for its global loads, which stay in flight across it
+// (__syncthreads() waits for every outstanding memory operation).  Used where no
+// thread reads global data that another thread of the workgroup has written.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -208,16 +218,18 @@ __device__ inline void acc_add(const d4 (&acc)[4][4], double* buf, int lane) {
 // MODE 0: FP64 throughout; 1: descendant (kind 0) tiles on the FP32 copy,
 // intra-supernode tiles in FP64 (a tile's tasks all have one kind); 2: FP32
 // for every tile.
+// The body of one tile (a workgroup of kThreads threads; red: 2 x PW x kLdsLd
+// doubles of LDS), shared by k_tile_update and the dependency-driven kernel.
 template <int MODE>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
-k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
-              const Task* __restrict__ tasks, int tile0, double* __restrict__ dscratch,
-              const float* __restrict__ Lf) {
-  __shared__ double red[2][swc::PW * kLdsLd];
-  const Tile tl = tiles[tile0 + blockIdx.x];
+__device__ inline void tile_update_body(double* __restrict__ L, const Tile& tl,
+                                        const Task* __restrict__ tasks,
+                                        double* __restrict__ dscratch,
+                                        const float* __restrict__ Lf, double* red_lds,
+                                        int tid) {
+  double(*red)[swc::PW * kLdsLd] = reinterpret_cast<double(*)[swc::PW * kLdsLd]>(red_lds);
   // the wave index is uniform over the wave: say so, so that the per-chunk
   // buffer resources stay in scalar registers
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nr = tl.nr, nc = tl.ncol, ld = tl.ld;
   const bool diag = tl.r0 == 0;
   double* gtile = L + tl.vbase + tl.r0;
@@ -253,9 +265,9 @@ k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
   // partial sums (w0 + w2) and (w1 + w3) in LDS, added element-wise by all
   // threads below
   if (wave < 2) acc_store(acc, red[wave], lane);
-  __syncthreads();
+  lds_barrier();
   if (wave >= 2) acc_add(acc, red[wave - 2], lane);
-  __syncthreads();
+  lds_barrier();
   // write back tile - sum; rows 0..nc-1 of a tile at row offset 0 are the
   // panel's diagonal block, whose strict upper part stays untouched (zero).
   // When this level factors the panel, the diagonal block also goes to the
@@ -274,6 +286,16 @@ k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
       }
     }
   }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_tile_update(double* __restrict__ L, const Tile* __restrict__ tiles,
+              const Task* __restrict__ tasks, int tile0, double* __restrict__ dscratch,
+              const float* __restrict__ Lf) {
+  __shared__ double red[2 * swc::PW * kLdsLd];
+  const Tile tl = tiles[tile0 + blockIdx.x];
+  tile_update_body<MODE>(L, tl, tasks, dscratch, Lf, red, threadIdx.x);
 }
 
 // Tiles of panels of at most 16 columns with a short total inner dimension: one
@@ -349,12 +371,10 @@ __device__ inline double bcast(double v, int src) {
 // above the diagonal of the register copy are never read by lanes on or below
 // it, so they are left unmasked.
 template <int NC>
-__global__ void __launch_bounds__(64)
-k_panel_factor_regs(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
-                    const double* __restrict__ dscratch, int* __restrict__ info_out,
-                    float* __restrict__ Lf) {
-  const Tile tl = ftiles[list0 + blockIdx.x];
-  const int lane = threadIdx.x;
+__device__ inline void panel_factor_regs_body(double* __restrict__ L, const Tile& tl,
+                                              const double* __restrict__ dscratch,
+                                              int* __restrict__ info_out, float* __restrict__ Lf,
+                                              int lane) {
   const int nr = tl.nr, nc = tl.ncol;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
   double y[NC];                              // row `lane` of the diagonal block
@@ -409,6 +429,15 @@ k_panel_factor_regs(double* __restrict__ L, const Tile* __restrict__ ftiles, int
       gtile[(int64_t)c * tl.ld + lane] = x[c];
       if (ftile) ftile[(int64_t)c * tl.ld + lane] = (float)x[c];
     }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(64)
+k_panel_factor_regs(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
+                    const double* __restrict__ dscratch, int* __restrict__ info_out,
+                    float* __restrict__ Lf) {
+  const Tile tl = ftiles[list0 + blockIdx.x];
+  panel_factor_regs_body<NC>(L, tl, dscratch, info_out, Lf, threadIdx.x);
 }
 
 
@@ -592,14 +621,16 @@ __device__ inline d4 lds_block_update(const double* __restrict__ W, d4 acc, int 
 // bitwise the one of a column-by-column right-looking factorization.  Every
 // tile of a panel computes the same L11, so no tile waits for another; the
 // diagonal tile stores it and reports info.
-__global__ void __launch_bounds__(256)
-k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
-                   const double* __restrict__ dscratch, int* __restrict__ info_out,
-                   float* __restrict__ Lf) {
-  __shared__ double W[swc::PW * kWld];
-  __shared__ double lc[3][16][16];  // per sweeping wave: column j's multipliers
-  const Tile tl = ftiles[list0 + blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+// LDS of k_panel_factor_blk: W, then per sweeping wave column j's multipliers
+constexpr int kBlkLds = swc::PW * kWld + 3 * 16 * 16;
+
+__device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile& tl,
+                                             const double* __restrict__ dscratch,
+                                             int* __restrict__ info_out, float* __restrict__ Lf,
+                                             double* lds, int tid) {
+  double* W = lds;
+  double(*lc)[16][16] = reinterpret_cast<double(*)[16][16]>(lds + swc::PW * kWld);
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nc = tl.ncol;
   const int nb = (nc + 15) >> 4;              // 16-column blocks (2..4)
   const int first = tl.kind == 0 ? nc : 0;    // tile rows [first, nr) are solved ...
@@ -630,7 +661,7 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
   int badcol = 0;
   for (int J = 0; J < nb; ++J) {
     const int jb = 16 * J;
@@ -644,15 +675,24 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
     double x[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) x[c] = W[(jb + c) * kWld + row];
-    __syncthreads();
+    lds_barrier();
     if (wave < 3) {
       const int pbase = wave == 0 ? jb : 0;
       const bool store = wave == 0 ? (lane >= jb && lane < 16 * nb)
                                    : (lane >= 16 && row - 64 < nx && (wave == 1 || lane < 32));
       double(*mult)[16] = lc[wave];
       double d = bcast(x[0], pbase);
+      // Software-pipelined over the columns: column j's multiplier for column
+      // j + 1 (the entry (j+1, j), needed by the next pivot) comes by readlane;
+      // its multipliers for the columns after that go through LDS and are
+      // applied one column later, so no LDS round trip sits on the pivot chain.
+      // Every entry still takes its updates in increasing column order.
+      double pm[16];  // column j-1's multipliers, read from LDS during column j-1
+#pragma unroll
+      for (int c = 0; c < 16; ++c) pm[c] = 0.0;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
+        // x[j] has every update from the columns before j
         if (d > 0.0) {
           // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
           // by 1/ajj)
@@ -663,11 +703,18 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
           x[j] = lane == pbase + j ? 1.0 : x[j];
         }
         if (j < 15) {
-          // the next pivot is its lane's x[j+1] after this column's update
-          d = bcast(x[j + 1] - x[j] * x[j], pbase + j + 1);
+          const double m1 = bcast(x[j], pbase + j + 1);  // entry (j+1, j)
+          if (j >= 1) x[j + 1] -= x[j - 1] * pm[j + 1];
+          x[j + 1] -= x[j] * m1;
+          // the next pivot: its lane's x[j+1], now updated by every earlier column
+          d = bcast(x[j + 1], pbase + j + 1);
           if (lane >= pbase && lane < pbase + 16) mult[j][lane - pbase] = x[j];
+          if (j >= 1) {
 #pragma unroll
-          for (int c = j + 1; c < 16; ++c) x[c] -= x[j] * mult[j][c];
+            for (int c = j + 2; c < 16; ++c) x[c] -= x[j - 1] * pm[c];
+          }
+#pragma unroll
+          for (int c = j + 2; c < 16; ++c) pm[c] = mult[j][c];
         }
       }
       if (store) {
@@ -675,7 +722,7 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
         for (int c = 0; c < 16; ++c) W[(jb + c) * kWld + row] = x[c];
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (J + 1 < nb) {
       // (2) block columns C > J: rows of the diagonal block from C on, then the
       // tile rows (W row blocks 4 .. 4 + nxb)
@@ -694,7 +741,7 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
           out[12] = acc[3];
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
   if (tl.kind == 0 && badcol && badcol <= nc && tid == 0) atomicMin(info_out, tl.c0 + badcol);
@@ -715,6 +762,144 @@ k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int 
         if (ftile) ftile[(int64_t)col * tl.ld + first + lane] = (float)v;
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
+                   const double* __restrict__ dscratch, int* __restrict__ info_out,
+                   float* __restrict__ Lf) {
+  __shared__ double lds[kBlkLds];
+  const Tile tl = ftiles[list0 + blockIdx.x];
+  panel_factor_blk_body(L, tl, dscratch, info_out, Lf, lds, threadIdx.x);
+}
+
+// ---- dependency-driven factorization ----------------------------------------
+// One persistent launch for the plan's levels from dag_level0 on (swc::DagItem):
+// as many workgroups as the device holds at once claim work items from a ticket
+// counter, in the plan's item order.  An item applies one level's updates to one
+// tile with tile_update_body and, when that level completes the tile's panel,
+// factors the tile's rows with the panel factor bodies; the arithmetic is that of
+// the per-level launches, so the factor is bitwise the same.  Before it starts, an
+// item waits for the tile's previous writer and for every panel its updates read
+// to be factored (all of the panel's tiles); a factoring tile below the diagonal
+// also waits for its panel's diagonal tile to be updated (the copy of the
+// diagonal block it factors from).  Item order is a topological order of these
+// waits, so an item only waits for items claimed earlier by running workgroups
+// that wait for nothing later: the launch cannot deadlock.  Every wait is also
+// bounded (kDagWaitTicks of the constant 100 MHz clock); a wait that runs out
+// sets the abort flag, every workgroup then stops, and the host reports the
+// failure.  Publication: every wave waits for its stores to complete, a barrier,
+// then one thread's agent-scope release store of the flag (which writes the
+// XCD's cache back first: the device's L2 caches are per XCD); the waiting side
+// polls with relaxed agent-scope loads, then the polling wave takes an
+// agent-scope acquire fence (invalidating the CU's and the XCD's caches) before
+// the barrier after which the workgroup loads.
+// sync: [0] ticket counter, [1] abort flag, [2, 2 + nitems) item states (1: the
+// tile's updates are written), then per panel the count of its factored tiles.
+constexpr int kDagLds = kBlkLds > 2 * swc::PW * kLdsLd ? kBlkLds : 2 * swc::PW * kLdsLd;
+constexpr uint64_t kDagWaitTicks = 500000000ull;  // 5 s
+
+__device__ inline int ld_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One wave polls until the tile's previous writer (prev >= 0) has written it
+// and the panels deps[dep0, dep1) are factored; false when the launch aborts.
+__device__ bool dag_poll(const int* __restrict__ state, const int* __restrict__ pdone,
+                         const swc::DagDep* __restrict__ deps, int prev, int dep0, int dep1,
+                         int* __restrict__ abort_flag, int lane) {
+  const uint64_t t0 = wall_clock64();
+  for (;;) {
+    bool ok = !(lane == 0 && prev >= 0) || ld_agent(state + prev) >= 1;
+    for (int e = dep0 + lane; e < dep1; e += 64) {
+      const swc::DagDep dp = deps[e];
+      ok = ok && ld_agent(pdone + dp.panel) >= dp.count;
+    }
+    if (__builtin_amdgcn_ballot_w64(!ok) == 0) return true;
+    if (ld_agent(abort_flag) != 0) return false;
+    if (wall_clock64() - t0 > kDagWaitTicks) {
+      if (lane == 0) __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_dag_factor(double* __restrict__ L, const Tile* __restrict__ tiles,
+             const Task* __restrict__ tasks, const swc::DagItem* __restrict__ items, int nitems,
+             const swc::DagDep* __restrict__ deps, int* __restrict__ sync,
+             double* __restrict__ dscratch, int* __restrict__ info_out, float* Lf, int write_f32) {
+  __shared__ double lds[kDagLds];
+  __shared__ int s_claim, s_ok;
+  int* const ticket = sync;
+  int* const abort_flag = sync + 1;
+  int* const state = sync + 2;
+  int* const pdone = sync + 2 + nitems;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float* const lf = write_f32 ? Lf : nullptr;
+  for (;;) {
+    if (tid == 0) {
+      const int c = atomicAdd(ticket, 1);
+      s_claim = ld_agent(abort_flag) != 0 ? nitems : c;
+    }
+    __syncthreads();
+    const int k = s_claim;
+    if (k >= nitems) break;
+    {
+      const swc::DagItem d = items[k];
+      if (wave == 0) {
+        const bool ok = dag_poll(state, pdone, deps, d.prev, d.dep0, d.dep1, abort_flag, lane);
+        // one acquire for the workgroup: it invalidates the CU's and the XCD's
+        // caches, and the other waves load only after the barrier below
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (lane == 0) s_ok = ok;
+      }
+      __syncthreads();
+      if (!s_ok) break;
+      // the thread index is made opaque per item, so that the compiler does not
+      // hoist the bodies' per-lane address arithmetic out of the loop and keep it
+      // in registers across items
+      int t1 = tid;
+      asm volatile("" : "+v"(t1));
+      tile_update_body<MODE>(L, tiles[d.tile], tasks, dscratch, Lf, lds, t1);
+    }
+    // every wave's stores complete before the barrier; the one release below then
+    // writes them back from the XCD's cache before the flag is set
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // the item and its tile are read again rather than kept in registers over the
+    // updates (an index the compiler cannot see through)
+    int k2 = k;
+    asm volatile("" : "+s"(k2));
+    const swc::DagItem d = items[k2];
+    const Tile tl = tiles[d.tile];
+    if (d.fclass < 0 || tl.kind == 0) {  // someone waits for these updates
+      if (tid == 0) __hip_atomic_store(state + k2, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (d.fclass >= 0) {
+      if (wave == 0) {
+        const bool ok = d.diag < 0 || dag_poll(state, pdone, deps, d.diag, 0, 0, abort_flag, lane);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (lane == 0) s_ok = ok;
+      }
+      __syncthreads();
+      if (!s_ok) break;
+      int t2 = tid;
+      asm volatile("" : "+v"(t2));
+      if (d.fclass == 0) {
+        if (wave == 0) panel_factor_regs_body<16>(L, tl, dscratch, info_out, lf, t2 & 63);
+      } else {
+        panel_factor_blk_body(L, tl, dscratch, info_out, lf, lds, t2);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_fetch_add(pdone + tl.panel, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
   }
 }
 
@@ -844,10 +1029,24 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   const double* D = L + sp.vbase;  // the panel's diagonal block, column j at D + j * ld
   const int rb0 = blockIdx.y * RB, nq = min(RB, nrhs - rb0);  // this group's right-hand sides
   // xs[q][c] = x_q of the panel's columns (zero for padded right-hand sides)
+  // The right-hand sides are loaded into registers before the panel's entries
+  // and written to LDS after them: loads complete in issue order, so this way
+  // the write waits only for its own loads, and the panel's entries arrive while
+  // the substitution runs (the barriers below wait for LDS traffic only).
+  constexpr int NX = (RB * swc::SPW + kThreads - 1) / kThreads;
+  double xv[NX];
+#pragma unroll
+  for (int u = 0; u < NX; ++u) {
+    const int e = tid + u * kThreads, q = e / swc::SPW, c = e % swc::SPW;
+    const bool in = e < RB * swc::SPW && c < nc && q < nq;
+    const double v = X[in ? (int64_t)(rb0 + q) * ldx + sp.c0 + c : 0];
+    xv[u] = in ? v : 0.0;
+  }
   auto stage = [&]() {
-    for (int e = tid; e < RB * swc::SPW; e += kThreads) {
-      const int q = e / swc::SPW, c = e % swc::SPW;
-      if (c < nc) xs[q][c] = q < nq ? X[(int64_t)(rb0 + q) * ldx + sp.c0 + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NX; ++u) {
+      const int e = tid + u * kThreads, q = e / swc::SPW, c = e % swc::SPW;
+      if (e < RB * swc::SPW && c < nc) xs[q][c] = xv[u];
     }
   };
   auto store = [&]() {
@@ -859,16 +1058,18 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
   if (nc <= 64) {
     // one chunk: wave 0 substitutes it, then the block's rows are pushed
     stage();
-    __syncthreads();
+    lds_barrier();
     if (wave == 0) {
+      // row `lane` (clamped into the diagonal block), columns uniform over the wave
+      const int rl = min(lane, nc - 1);
+      const double dval = D[(int64_t)rl * sp.ld + rl];
       double row[64];
 #pragma unroll
       for (int c = 0; c < 64; ++c) {
         const bool in = lane < nc && c < lane;
-        const double val = D[in ? (int64_t)c * sp.ld + lane : 0];
+        const double val = (D + (int64_t)min(c, nc - 1) * sp.ld)[rl];
         row[c] = in ? val : 0.0;
       }
-      const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
       const double rinv = lane < nc ? 1.0 / dval : 1.0;
       double ti[RB];
 #pragma unroll
@@ -879,54 +1080,62 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
         for (int q = 0; q < RB; ++q) xs[q][lane] = ti[q];
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (bk.writer) store();
     if (bk.nr > 0) {
       double acc[RB] = {};
+      const int rb = bk.r0 + min(lane, bk.nr - 1);  // row clamped into the block
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         const int j = wave + 4 * k;
         const bool in = lane < bk.nr && j < nc;
-        const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
+        const double val = (D + (int64_t)min(j, nc - 1) * sp.ld)[rb];
         const double lv = in ? val : 0.0;
 #pragma unroll
         for (int q = 0; q < RB; ++q) acc[q] = fma(lv, j < nc ? xs[q][j] : 0.0, acc[q]);
       }
 #pragma unroll
       for (int q = 0; q < RB; ++q) part[wave][q][lane] = acc[q];
-      __syncthreads();
+      lds_barrier();
       push_rows<RB>(part, sp, bk, rb0, nq, X, ldx, V, vld);
     }
     return;
   }
-  // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
-  // (entries outside the block read the panel's first diagonal entry, which
-  // exists even when the block is empty, and are replaced by zero)
-  constexpr int NQ = swc::SPW / 4;
-  double bv[NQ];
-#pragma unroll
-  for (int k = 0; k < NQ; ++k) {
-    const int j = wave + 4 * k;
-    const bool in = lane < bk.nr && j < nc;
-    const double val = D[in ? (int64_t)j * sp.ld + bk.r0 + lane : 0];
-    bv[k] = in ? val : 0.0;
-  }
+  // Loads are issued in the order they are needed (they complete in issue
+  // order): this wave's diagonal entry, its step's 64 columns, then its share of
+  // the block's rows, which are used last.
   // this wave's step: element (row0 + lane, col0 + c), c < 64, where valid
+  // Every load takes a column base that is uniform over the wave (scalar
+  // arithmetic) and a row clamped into the panel once per lane; entries outside
+  // the wanted range are read from a valid address and replaced by zero.
   const int row0 = wave == 0 ? 0 : 64, col0 = wave == 2 ? 64 : 0;
   const int nrow = wave == 0 ? na0 : na1, ncol = wave == 2 ? na1 : na0;
+  const int rlast = sp.nrow - 1;  // last row of the panel's columns
+  const bool dg = (wave == 0 || wave == 2) && lane < nrow;
+  const int rs = min(row0 + lane, rlast);  // this lane's row of the step
+  const double dval = D[(int64_t)min(row0 + lane, nc - 1) * sp.ld + rs];
   double m[64];
 #pragma unroll
   for (int c = 0; c < 64; ++c) {
     const bool in = wave < 3 && lane < nrow && c < ncol && (wave == 1 || c < lane);
-    const double val = D[in ? (int64_t)(col0 + c) * sp.ld + row0 + lane : 0];
+    const double val = (D + (int64_t)min(col0 + c, nc - 1) * sp.ld)[rs];
     m[c] = in ? val : 0.0;
   }
-  const bool dg = (wave == 0 || wave == 2) && lane < nrow;
-  const double dval = D[dg ? (int64_t)(row0 + lane) * sp.ld + row0 + lane : 0];
+  // the block's rows: wave w holds columns j = w, w+4, ... of row `lane`
+  constexpr int NQ = swc::SPW / 4;
+  double bv[NQ];
+  const int rb = min(bk.r0 + lane, rlast);
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int j = wave + 4 * k;
+    const bool in = lane < bk.nr && j < nc;
+    const double val = (D + (int64_t)min(j, nc - 1) * sp.ld)[rb];
+    bv[k] = in ? val : 0.0;
+  }
   // reciprocal of the diagonal entry; 1 beyond the chunk (padded unknowns stay 0)
   const double rinv = dg ? 1.0 / dval : 1.0;
   stage();
-  __syncthreads();
+  lds_barrier();
   if (wave == 0) {
     double ti[RB];
 #pragma unroll
@@ -937,7 +1146,7 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
       for (int q = 0; q < RB; ++q) xs[q][lane] = ti[q];
     }
   }
-  __syncthreads();
+  lds_barrier();
   if (na1 > 0) {
     if (wave == 1 && lane < na1) {
 #pragma unroll
@@ -949,7 +1158,7 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (wave == 2) {
       double ti[RB];
 #pragma unroll
@@ -960,7 +1169,7 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
         for (int q = 0; q < RB; ++q) xs[q][64 + lane] = ti[q];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (bk.writer) store();
   if (bk.nr > 0) {
@@ -979,7 +1188,7 @@ k_fwd_panel(const double* __restrict__ L, const Panel* __restrict__ spanels,
       part[wave][q][lane] = acc;
       __builtin_amdgcn_sched_barrier(0);
     }
-    __syncthreads();
+    lds_barrier();
     push_rows<RB>(part, sp, bk, rb0, nq, X, ldx, V, vld);
   }
 }
@@ -1049,7 +1258,7 @@ k_bwd_partial_narrow(const double* __restrict__ L, const int* __restrict__ Ls,
 #pragma unroll
   for (int c = 0; c < NW; ++c)
     pr[lane][c] = (lane < bk.nr && c < nc) ? blk[(int64_t)c * sp.ld + lane] * xr : 0.0;
-  __syncthreads();
+  lds_barrier();
   const int cl = lane % NW, g = lane / NW;
   double sum = 0.0;
 #pragma unroll
@@ -1117,21 +1326,33 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
   const int rb0 = blockIdx.y * RB, nq = min(RB, nrhs - rb0);  // this group's right-hand sides
   // column c, rows [H h, H h + H) of the block (entries outside it read the
   // panel's first diagonal entry, which always exists, and are replaced by zero)
-  const double* D = L + sp.vbase;
-  const int64_t off = (int64_t)c * sp.ld + bk.r0 + H * h;
+  // (the column clamped into the panel; a full block reads its 32 rows from one
+  // base address, a partial one clamps each row into the block)
+  const double* colp = L + sp.vbase + (int64_t)min(c, nc - 1) * sp.ld + bk.r0;
   double lv[H];
+  if (nr == swc::TM) {
 #pragma unroll
-  for (int i = 0; i < H; ++i) {
-    const bool in = c < nc && H * h + i < nr;
-    const double val = D[in ? off + i : 0];
-    lv[i] = in ? val : 0.0;
+    for (int i = 0; i < H; ++i) {
+      const double val = colp[H * h + i];
+      lv[i] = c < nc ? val : 0.0;
+    }
+  } else if (nr > 0) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const bool in = c < nc && H * h + i < nr;
+      const double val = colp[min(H * h + i, nr - 1)];
+      lv[i] = in ? val : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < H; ++i) lv[i] = 0.0;
   }
   for (int e = tid; e < RB * swc::TM; e += kThreads) {
     const int q = e / swc::TM, i = e % swc::TM;
     if (i < nr)
       xr[q][i] = q < nq ? X[(int64_t)(rb0 + q) * ldx + Ls[sp.rows_off + bk.r0 + i]] : 0.0;
   }
-  __syncthreads();
+  lds_barrier();
   // one right-hand side at a time (the barrier bounds the LDS reads in flight
   // to what the registers hold)
 #pragma unroll
@@ -1142,7 +1363,7 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
     part[h][q][c] = acc;
     __builtin_amdgcn_sched_barrier(0);
   }
-  __syncthreads();
+  lds_barrier();
   const int64_t slot = (int64_t)(b0 + blockIdx.x - bl) * swc::SPW;
   for (int e = tid; e < RB * swc::SPW; e += kThreads) {
     const int q = e / swc::SPW, cc = e % swc::SPW;
@@ -1186,7 +1407,7 @@ __device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& 
 #pragma unroll
     for (int q = 0; q < RB; ++q) ts[q][c] = sum[q];
   }
-  __syncthreads();
+  lds_barrier();
   if (h == 0 && c < nc) {
 #pragma unroll
     for (int q = 0; q < RB; ++q)
@@ -1228,17 +1449,19 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
     double col[64];
     double rinv = 1.0;
     if (wave == 0) {
+      // column `lane` (clamped into the panel), rows uniform over the wave
+      const double* colp = D + (int64_t)min(lane, nc - 1) * sp.ld;
+      const double dval = colp[min(lane, nc - 1)];
 #pragma unroll
       for (int j = 0; j < 64; ++j) {
         const bool in = lane < nc && j > lane && j < nc;
-        const double val = D[in ? (int64_t)lane * sp.ld + j : 0];
+        const double val = colp[min(j, nc - 1)];
         col[j] = in ? val : 0.0;
       }
-      const double dval = D[lane < nc ? (int64_t)lane * sp.ld + lane : 0];
       rinv = lane < nc ? 1.0 / dval : 1.0;
     }
     gather_partials<RB>(ts, sp, f0, f1, b0, Pb, pld, Y, ldy, rb0, nq);
-    __syncthreads();
+    lds_barrier();
     if (wave == 0) {
       double ti[RB];
 #pragma unroll
@@ -1249,25 +1472,28 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
         for (int q = 0; q < RB; ++q) ts[q][lane] = ti[q];
       }
     }
-    __syncthreads();
+    lds_barrier();
     store();
     return;
   }
   // this wave's step: m[j] = L(rowc + j, colc + lane) where valid
   const int colc = wave == 0 ? 64 : 0, rowc = wave == 2 ? 0 : 64;
   const int ncl = wave == 0 ? na1 : na0, nrw = wave == 2 ? na0 : na1;
+  // column colc + lane (clamped into the panel), rows uniform over the wave;
+  // the diagonal entry first, as loads complete in issue order
+  const double* colp = D + (int64_t)min(colc + lane, nc - 1) * sp.ld;
+  const bool dg = (wave == 0 || wave == 2) && lane < ncl;
+  const double dval = colp[min(colc + lane, nc - 1)];
   double m[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) {
     const bool in = wave < 3 && lane < ncl && j < nrw && (wave == 1 || j > lane);
-    const double val = D[in ? (int64_t)(colc + lane) * sp.ld + rowc + j : 0];
+    const double val = colp[min(rowc + j, nc - 1)];
     m[j] = in ? val : 0.0;
   }
-  const bool dg = (wave == 0 || wave == 2) && lane < ncl;
-  const double dval = D[dg ? (int64_t)(colc + lane) * sp.ld + colc + lane : 0];
   const double rinv = dg ? 1.0 / dval : 1.0;
   gather_partials<RB>(ts, sp, f0, f1, b0, Pb, pld, Y, ldy, rb0, nq);
-  __syncthreads();
+  lds_barrier();
   if (na1 > 0) {
     if (wave == 0) {
       double ti[RB];
@@ -1279,7 +1505,7 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
         for (int q = 0; q < RB; ++q) ts[q][64 + lane] = ti[q];
       }
     }
-    __syncthreads();
+    lds_barrier();
     // first-chunk entries: t_i -= sum_j L(64 + j, i) x_(64 + j); ts is zero
     // past nc (gather_partials writes every column), so no read is masked
     if (wave == 1) {
@@ -1292,7 +1518,7 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (wave == 2) {
     double ti[RB];
@@ -1304,7 +1530,7 @@ k_bwd_diag(const double* __restrict__ L, const Panel* __restrict__ spanels,
       for (int q = 0; q < RB; ++q) ts[q][lane] = ti[q];
     }
   }
-  __syncthreads();
+  lds_barrier();
   store();
 }
 
@@ -1748,6 +1974,10 @@ struct swc_handle {
   swc::FusedPanel* dFused = nullptr;  // fused narrow panels, by level
   int* dCounters = nullptr;  // per level: the fused kernel's worklist counter
   int fused_cap[3] = {0, 0, 0};  // per precision mode: fused-kernel workgroups resident at once
+  swc::DagItem* dDagItems = nullptr;  // dependency-driven launch: work items, dependencies,
+  swc::DagDep* dDagDeps = nullptr;    // and its ticket / abort / per-item / per-panel flags
+  int* dDagSync = nullptr;
+  int dag_cap[3] = {0, 0, 0};  // per precision mode: its workgroups resident at once
   double* dDiag = nullptr;  // per-level copies of the diagonal blocks being factored
   int64_t* dAmap = nullptr;
   int* dPerm = nullptr;
@@ -1821,14 +2051,14 @@ struct swc_handle {
   void release() {
     drop_graphs();
     (void)hipFree(dL); (void)hipFree(dLf); (void)hipFree(dLs); (void)hipFree(dPanels); (void)hipFree(dTiles); (void)hipFree(dTasks);
-    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dFtiles); (void)hipFree(dFused); (void)hipFree(dCounters); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
+    (void)hipFree(dGptr); (void)hipFree(dGidx); (void)hipFree(dGcols); (void)hipFree(dFblk); (void)hipFree(dFblkPtr); (void)hipFree(dV); (void)hipFree(dPb); (void)hipFree(dY); (void)hipFree(dSpanels); (void)hipFree(dSpOrder); (void)hipFree(dOrder); (void)hipFree(dFtiles); (void)hipFree(dFused); (void)hipFree(dCounters); (void)hipFree(dDagItems); (void)hipFree(dDagDeps); (void)hipFree(dDagSync); (void)hipFree(dDiag); (void)hipFree(dAmap); (void)hipFree(dPerm);
     (void)hipFree(dVals); (void)hipFree(dWork); (void)hipFree(dInfo);
     (void)hipFree(dRp); (void)hipFree(dCi); (void)hipFree(dVmap); (void)hipFree(dAf); (void)hipFree(dNorm); (void)hipFree(dScal); (void)hipFree(dPart); (void)hipFree(dNormPart);
     (void)hipFree(dRef);
     (void)hipFree(dZ); (void)hipFree(dArena); (void)hipFree(dNodes); (void)hipFree(dCol2sup); (void)hipFree(dSpx);
     (void)hipFree(dSsuper); (void)hipFree(dSpi); (void)hipFree(dDiagPos); (void)hipFree(dQi); (void)hipFree(dQj);
     dL = nullptr; dLf = nullptr; dLs = nullptr; dPanels = nullptr; dTiles = nullptr; dTasks = nullptr;
-    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dFtiles = nullptr; dFused = nullptr; dCounters = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
+    dGptr = nullptr; dGidx = nullptr; dGcols = nullptr; dFblk = nullptr; dFblkPtr = nullptr; dV = nullptr; dPb = nullptr; dY = nullptr; dSpanels = nullptr; dSpOrder = nullptr; dOrder = nullptr; dFtiles = nullptr; dFused = nullptr; dCounters = nullptr; dDagItems = nullptr; dDagDeps = nullptr; dDagSync = nullptr; dDiag = nullptr; dAmap = nullptr; dPerm = nullptr;
     dVals = nullptr; dWork = nullptr; dInfo = nullptr;
     dRp = nullptr; dCi = nullptr; dVmap = nullptr; dAf = nullptr; dNorm = nullptr; dScal = nullptr; dPart = nullptr; dNormPart = nullptr;
     lf_cap = 0;
@@ -1933,7 +2163,14 @@ void enqueue_factor(swc_handle* h, bool profile) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, h->dInfo, 0x7fffffff);
   hipLaunchKernelGGL(k_fill_int, dim3((std::max(P.nlevel, 1) + kThreads - 1) / kThreads),
                      dim3(kThreads), 0, st, h->dCounters, std::max(P.nlevel, 1), 0);
-  for (int l = 0; l < P.nlevel; ++l) {
+  // the levels from dag_level0 on run as one dependency-driven launch (per-level
+  // launches when profiling, so that each level can be timed)
+  const bool dag = !profile && !P.dag_items.empty();
+  const int nsync = 2 + (int)P.dag_items.size() + P.npanel;
+  if (dag)
+    hipLaunchKernelGGL(k_fill_int, dim3((nsync + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                       h->dDagSync, nsync, 0);
+  for (int l = 0; l < (dag ? P.dag_level0 : P.nlevel); ++l) {
     const int t0 = P.lvl_tiles[l], nt = P.lvl_tiles[l + 1] - t0;
     const int s0 = P.lvl_trsm[l], ns = P.lvl_trsm[l + 1] - s0;
     hipEvent_t* pe = profile ? &h->pev[(size_t)4 * l] : nullptr;
@@ -1980,6 +2217,16 @@ void enqueue_factor(swc_handle* h, bool profile) {
     }
     if (pe) (void)hipEventRecord(pe[2], st);
   }
+  if (dag) {
+    auto kern = h->precision == 0 ? k_dag_factor<0> : (h->precision == 1 ? k_dag_factor<1>
+                                                                          : k_dag_factor<2>);
+    const int nitems = (int)P.dag_items.size();
+    const int grid = std::min(nitems, h->dag_cap[h->precision]);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, h->dL, (const Tile*)h->dTiles,
+                       (const Task*)h->dTasks, (const swc::DagItem*)h->dDagItems, nitems,
+                       (const swc::DagDep*)h->dDagDeps, h->dDagSync, h->dDiag, h->dInfo, h->dLf,
+                       h->fp32_mask ? 1 : 0);
+  }
 }
 
 // Numeric factorization from values on the device (in upload order).
@@ -2019,8 +2266,16 @@ int factor_device(swc_handle* h, const double* dvals) {
   hipError_t ce = hipMemcpyAsync(&info, h->dInfo, sizeof(int), hipMemcpyDeviceToHost, st);
   if (ce == hipSuccess)
     ce = hipMemcpyAsync(&h->normA, h->dNorm, sizeof(double), hipMemcpyDeviceToHost, st);
+  int dag_abort = 0;
+  const bool dag = !h->profile && !P.dag_items.empty();
+  if (ce == hipSuccess && dag)
+    ce = hipMemcpyAsync(&dag_abort, h->dDagSync + 1, sizeof(int), hipMemcpyDeviceToHost, st);
   SWC_HIP(hipStreamSynchronize(st));
   SWC_HIP(ce);
+  if (dag_abort) {
+    h->factored = false;
+    return swc_set_error(SWC_EDEVICE, "factor: a dependency wait of the factor launch timed out");
+  }
   float ms = 0;
   SWC_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
   h->t_factor_ms = ms;
@@ -2540,6 +2795,21 @@ int upload_structure(swc_handle* h, int n, const int* blockSet, int supNo, const
       rc = swc_set_error(SWC_EDEVICE, std::string("upload: occupancy query: ") + hipGetErrorString(e));
     else
       for (int m = 0; m < 3; ++m) h->fused_cap[m] = std::max(1, per[m]) * std::max(ncu, 1);
+  }
+  if (!rc) rc = upload(&h->dDagItems, P.dag_items.data(), P.dag_items.size(), st);
+  if (!rc) rc = upload(&h->dDagDeps, P.dag_deps.data(), P.dag_deps.size(), st);
+  if (!rc)
+    rc = upload(&h->dDagSync, (const int*)nullptr, 2 + P.dag_items.size() + (size_t)P.npanel, st);
+  if (!rc && h->dag_cap[0] == 0) {
+    int ncu = 0, per[3] = {0, 0, 0};
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_dag_factor<0>, kThreads, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_dag_factor<1>, kThreads, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_dag_factor<2>, kThreads, 0);
+    if (e != hipSuccess)
+      rc = swc_set_error(SWC_EDEVICE, std::string("upload: occupancy query: ") + hipGetErrorString(e));
+    else
+      for (int m = 0; m < 3; ++m) h->dag_cap[m] = std::max(1, per[m]) * std::max(ncu, 1);
   }
   if (!rc) rc = upload(&h->dDiag, (const double*)nullptr, (size_t)P.diag_scratch, st);
   if (!rc) rc = upload(&h->dNorm, (const double*)nullptr, 8, st);
@@ -3066,9 +3336,14 @@ int swc_plan_stats(const swc_handle* h, int64_t* out, double* flops) {
   out[2] = (int64_t)P.tasks.size();
   out[3] = P.nlevel;
   out[4] = P.xsize;
+  out[5] = (int64_t)P.dag_items.size();
+  out[6] = P.dag_items.empty() ? P.nlevel : P.dag_level0;
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
+    flops[2] = P.dag_flops;
+    flops[3] = P.dag_sim_us;
+    flops[4] = P.lvl_sim_us;
   }
   return SWC_OK;
 }

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <queue>
 #include <stdexcept>
 #include <string>
 
@@ -16,6 +17,199 @@ namespace {
 
 int lower_idx(const int* a, int len, int v) { return int(std::lower_bound(a, a + len, v) - a); }
 int upper_idx(const int* a, int len, int v) { return int(std::upper_bound(a, a + len, v) - a); }
+
+// Work items of the dependency-driven factor launch (DagItem) for the tiles of
+// levels dag_level0 on, and the order in which the launch's workgroups claim
+// them.  Any topological order of the waits keeps the launch free of deadlock:
+// an item waits only for items claimed before it, by workgroups that are running
+// and never wait for anything later.
+//  mode 2: level order; in each level the tiles of the panels it factors first,
+//          each panel's diagonal tile before the others.
+//  mode 1: the order in which a list schedule on the device's resident
+//          workgroups would start the items, simulated here with a cost model:
+//          whenever a workgroup is free it takes the ready item with the longest
+//          remaining path (its "bottom level"), so the chain of a wide
+//          supernode's panel factorizations is not held up behind updates that
+//          can wait.  Costs are in microseconds: a fixed part plus the useful
+//          update flops at about 0.1 TFLOP/s per workgroup, plus the panel
+//          factorization.
+void build_dag(Plan& P, const std::vector<int>& task_src, const std::vector<int>& tile_level,
+               const std::vector<double>& tile_flops, const std::vector<char>& is_factor,
+               int mode) {
+  const int t0 = P.lvl_tiles[P.dag_level0], nt = (int)P.tiles.size();
+  std::vector<int> base;
+  for (int ti = t0; ti < nt; ++ti) base.push_back(ti);
+  auto group = [&](int ti) { return is_factor[ti] ? (P.tiles[ti].kind == 0 ? 0 : 1) : 2; };
+  std::stable_sort(base.begin(), base.end(), [&](int a, int b) {
+    if (tile_level[a] != tile_level[b]) return tile_level[a] < tile_level[b];
+    return group(a) < group(b);
+  });
+  const int N = (int)base.size();
+  std::vector<int> nfac(P.npanel, 0);  // factoring tiles of each panel
+  for (int ti = t0; ti < nt; ++ti)
+    if (is_factor[ti]) nfac[P.tiles[ti].panel]++;
+  std::vector<DagItem> it(N);
+  std::vector<std::vector<int>> deps(N);
+  std::vector<int64_t> rb0(P.npanel + 1, 0);  // row blocks of each panel
+  for (int q = 0; q < P.npanel; ++q) rb0[q + 1] = rb0[q] + (P.panels[q].nrow + TM - 1) / TM;
+  std::vector<int> last((size_t)rb0[P.npanel], -1), diag_item(P.npanel, -1);
+  for (int k = 0; k < N; ++k) {
+    const int ti = base[k];
+    const Tile& tl = P.tiles[ti];
+    DagItem& d = it[k];
+    d.tile = ti;
+    d.fclass = is_factor[ti] ? (tl.ncol <= 16 ? 0 : 1) : -1;
+    int& lt = last[(size_t)(rb0[tl.panel] + tl.r0 / TM)];
+    d.prev = lt;
+    lt = k;
+    d.diag = -1;
+    if (is_factor[ti]) {
+      if (tl.kind == 0) {
+        diag_item[tl.panel] = k;
+      } else {
+        d.diag = diag_item[tl.panel];
+        if (d.diag < 0) throw std::logic_error("factoring tile before its diagonal tile");
+      }
+    }
+    for (int t = tl.task_begin; t < tl.task_end; ++t) {
+      const int q = task_src[t];
+      if (P.panel_level[q] >= tile_level[ti]) throw std::logic_error("update reads a later panel");
+      if (P.panel_level[q] >= P.dag_level0) deps[k].push_back(q);
+    }
+    std::sort(deps[k].begin(), deps[k].end());
+    deps[k].erase(std::unique(deps[k].begin(), deps[k].end()), deps[k].end());
+  }
+  std::vector<int> order(N);
+  for (int k = 0; k < N; ++k) order[k] = k;
+  const double rate = 1e5;  // useful flops per microsecond and workgroup
+  auto c_upd = [&](int k) { return 3.0 + 1.4 * tile_flops[it[k].tile] / rate; };
+  auto c_fac = [&](int k) { return it[k].fclass < 0 ? 0.0 : (it[k].fclass == 0 ? 5.0 : 14.0); };
+  {
+    // the same levels as launches: per level, the updates (longest first on 512
+    // workgroups), then the factorizations, each launch with ~4 us of overhead
+    P.lvl_sim_us = 0;
+    for (int k0 = 0; k0 < N;) {
+      int k1 = k0;
+      while (k1 < N && tile_level[it[k1].tile] == tile_level[it[k0].tile]) ++k1;
+      std::vector<double> c;
+      double fac = 0;
+      for (int k = k0; k < k1; ++k) {
+        c.push_back(c_upd(k));
+        fac = std::max(fac, c_fac(k));
+      }
+      std::sort(c.rbegin(), c.rend());
+      std::priority_queue<double, std::vector<double>, std::greater<double>> w;
+      for (int i = 0; i < 512; ++i) w.push(0.0);
+      double span = 0;
+      for (double x : c) {
+        const double t = w.top() + x;
+        w.pop();
+        w.push(t);
+        span = std::max(span, t);
+      }
+      P.lvl_sim_us += span + 4.0 + (fac > 0 ? fac + 4.0 : 0.0);
+      k0 = k1;
+    }
+  }
+  if (mode == 1) {
+    std::vector<int> next_on_tile(N, -1);
+    for (int k = 0; k < N; ++k)
+      if (it[k].prev >= 0) next_on_tile[it[k].prev] = k;
+    // bottom levels, from the last item back (every successor comes later in base order)
+    std::vector<double> bottom(N, 0.0), pbot(P.npanel, 0.0), dbot(N, 0.0);
+    for (int k = N - 1; k >= 0; --k) {
+      double after = dbot[k];
+      if (next_on_tile[k] >= 0) after = std::max(after, bottom[next_on_tile[k]]);
+      if (it[k].fclass >= 0) after = std::max(after, c_fac(k) + pbot[P.tiles[it[k].tile].panel]);
+      bottom[k] = c_upd(k) + after;
+      for (int q : deps[k]) pbot[q] = std::max(pbot[q], bottom[k]);
+      if (it[k].diag >= 0) dbot[it[k].diag] = std::max(dbot[it[k].diag], bottom[k]);
+    }
+    std::vector<int> pending(N, 0), remaining(nfac);
+    std::vector<std::vector<int>> waiters(P.npanel), dwait(N);
+    for (int k = 0; k < N; ++k) {
+      pending[k] = (it[k].prev >= 0) + (it[k].diag >= 0) + (int)deps[k].size();
+      for (int q : deps[k]) waiters[q].push_back(k);
+      if (it[k].diag >= 0) dwait[it[k].diag].push_back(k);
+    }
+    typedef std::pair<double, int> Ready;  // (bottom level, -index): the largest first
+    std::priority_queue<Ready> ready;
+    struct Ev {
+      double t;
+      int type, k;  // type 0: updates written (diagonal tiles: releases the panel's tiles)
+      bool operator<(const Ev& o) const { return t > o.t || (t == o.t && k > o.k); }
+    };
+    std::priority_queue<Ev> ev;
+    for (int k = 0; k < N; ++k)
+      if (pending[k] == 0) ready.push({bottom[k], -k});
+    auto release = [&](int k) {
+      if (--pending[k] == 0) ready.push({bottom[k], -k});
+    };
+    const int workers = 512;  // resident workgroups of the launch: 256 CUs x 2
+    int free_w = workers, started = 0;
+    double now = 0;
+    while (started < N || !ev.empty()) {
+      while (free_w > 0 && !ready.empty()) {
+        const int k = -ready.top().second;
+        ready.pop();
+        order[started++] = k;
+        --free_w;
+        const double tu = now + c_upd(k);
+        if (it[k].fclass >= 0) {
+          ev.push({tu, 0, k});
+          ev.push({tu + c_fac(k), 1, k});
+        } else {
+          ev.push({tu, 1, k});
+        }
+      }
+      if (ev.empty()) throw std::logic_error("dependency simulation stalled");
+      const Ev e = ev.top();
+      ev.pop();
+      now = e.t;
+      if (e.type == 0) {
+        for (int d : dwait[e.k]) release(d);
+        continue;
+      }
+      ++free_w;
+      P.dag_sim_us = std::max(P.dag_sim_us, now);
+      if (next_on_tile[e.k] >= 0) release(next_on_tile[e.k]);
+      if (it[e.k].fclass >= 0) {
+        const int q = P.tiles[it[e.k].tile].panel;
+        if (--remaining[q] == 0)
+          for (int w : waiters[q]) release(w);
+      }
+    }
+  }
+  std::vector<int> pos(N), pmax(P.npanel, -1);
+  for (int k = 0; k < N; ++k) pos[order[k]] = k;
+  for (int k = 0; k < N; ++k)
+    if (it[k].fclass >= 0) {
+      int& m = pmax[P.tiles[it[k].tile].panel];
+      m = std::max(m, pos[k]);
+    }
+  P.dag_items.resize(N);
+  P.dag_deps.clear();
+  P.dag_flops = 0;
+  for (int k = 0; k < N; ++k) {
+    DagItem d = it[order[k]];
+    if (d.prev >= 0) d.prev = pos[d.prev];
+    if (d.diag >= 0) d.diag = pos[d.diag];
+    if (d.prev >= k || d.diag >= k) throw std::logic_error("item order is not topological");
+    d.dep0 = (int)P.dag_deps.size();
+    for (int q : deps[order[k]]) {
+      if (pmax[q] >= k) throw std::logic_error("item order is not topological");
+      P.dag_deps.push_back({q, nfac[q]});
+    }
+    d.dep1 = (int)P.dag_deps.size();
+    P.dag_items[k] = d;
+    P.dag_flops += tile_flops[d.tile];
+  }
+  for (int q = 0; q < P.npanel; ++q)
+    if (P.panel_level[q] >= P.dag_level0) {
+      const double w = P.panels[q].ncol, m = P.panels[q].nrow - P.panels[q].ncol;
+      P.dag_flops += w * w * w / 3.0 + m * w * w;
+    }
+}
 
 }  // namespace
 
@@ -63,9 +257,10 @@ bool plan_params_from(const int* v, int count, PlanParams& out) {
   if (count > 2) p.narrow_min_tiles = v[2];
   if (count > 3) p.narrow_max_k = v[3];
   if (count > 4) p.fuse_narrow = v[4];
+  if (count > 5) p.dag = v[5];
   if (count < 0 || count > kPlanParams || p.rlb < 1 || p.rlb > 64 ||
       (p.spw != 64 && p.spw != SPW) || p.narrow_min_tiles < 1 || p.narrow_max_k < 0 ||
-      (p.fuse_narrow != 0 && p.fuse_narrow != 1))
+      (p.fuse_narrow != 0 && p.fuse_narrow != 1) || p.dag < 0 || p.dag > 2)
     return false;
   out = p;
   return true;
@@ -175,6 +370,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   struct PTile { Tile t; int level; bool trsm; };
   std::vector<PTile> ptiles;
   std::vector<std::vector<Task>> ptasks;
+  std::vector<std::vector<int>> psrc;  // per task: the last source panel it reads
   P.panel_level.assign(P.npanel, 0);
   std::vector<int> sn_level(nsuper, -1);  // level at which a supernode is complete
   auto add_tile = [&](int q, int r0, int kind, int level, bool trsm) {
@@ -194,6 +390,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     pt.trsm = trsm;
     ptiles.push_back(pt);
     ptasks.emplace_back();
+    psrc.emplace_back();
     return (int)ptiles.size() - 1;
   };
   // descendant tasks of tile (panel q, rows r0..r0+nr) in pruneSet order
@@ -214,6 +411,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       t.ra = ra;
       t.rb = rb;
       ptasks[ti].push_back(t);
+      psrc[ti].push_back(first_panel[sr.d + 1] - 1);  // a supernode is final with its last panel
     }
   };
   // task of tile (panel q, rows r0..) from the earlier panels q2 .. q2+nsrc-1
@@ -228,6 +426,7 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     t.ra = off + tl.r0;
     t.rb = t.ra + tl.nr;
     ptasks[ti].push_back(t);
+    psrc[ti].push_back(q2 + nsrc - 1);
   };
 
   // Supernode s starts at the first level after its pruned descendants complete
@@ -454,6 +653,15 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   }
   auto fused = [&](int i) { return fused_panel[ptiles[i].t.panel] != 0; };
   auto narrow = [&](int i) { return is_narrow[i] != 0; };
+  // the dependency-driven launch takes the levels after the last one with fused
+  // or one-wave tiles (those keep their own launches)
+  P.dag_level0 = P.nlevel;
+  if (params.dag && factor) {
+    int last = -1;
+    for (size_t i = 0; i < ptiles.size(); ++i)
+      if (fused((int)i) || narrow((int)i)) last = std::max(last, ptiles[i].level);
+    P.dag_level0 = last + 1;
+  }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (ptiles[a].level != ptiles[b].level) return ptiles[a].level < ptiles[b].level;
     if (fused(a) != fused(b)) return fused(a);
@@ -475,9 +683,15 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   P.lvl_trsm.assign(P.nlevel + 1, 0);
   P.lvl_tile_flops.assign(P.nlevel, 0.0);
   std::vector<std::vector<int>> trsm_by_level(P.nlevel);
+  std::vector<int> task_src;  // per task of P.tasks: the last source panel it reads
+  std::vector<int> tile_level;
+  std::vector<double> tile_flops;  // per tile: useful flops of its updates
   for (int oi : order) {
     PTile& pt = ptiles[oi];
     pt.t.task_begin = (int)P.tasks.size();
+    tile_level.push_back(pt.level);
+    double tile_fl = 0;
+    task_src.insert(task_src.end(), psrc[oi].begin(), psrc[oi].end());
     const Panel& pn = P.panels[pt.t.panel];
     const int* trow = Ls + pn.rows_off + pt.t.r0;
     for (Task t : ptasks[oi]) {
@@ -506,7 +720,9 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       }
       P.tile_kernel_flops += 2.0 * pairs * t.k;
       P.lvl_tile_flops[pt.level] += 2.0 * pairs * t.k;
+      tile_fl += 2.0 * pairs * t.k;
     }
+    tile_flops.push_back(tile_fl);
     pt.t.task_end = (int)P.tasks.size();
     P.lvl_tiles[pt.level + 1]++;
     if (fused(oi)) {
@@ -523,10 +739,12 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   for (int l = 0; l < P.nlevel; ++l) P.lvl_fused[l + 1] += P.lvl_fused[l];
   for (int l = 0; l < P.nlevel; ++l) P.lvl_tiles[l + 1] += P.lvl_tiles[l];
   // scratch slots for the diagonal blocks each level factors: the update kernel
-  // copies the diagonal tile's block there, every factor-kernel tile reads it
+  // copies the diagonal tile's block there, every factor-kernel tile reads it.
+  // The levels of the dependency-driven launch are in flight together, so their
+  // slots do not overlap.
+  int64_t off = 0;
   for (int l = 0; l < P.nlevel; ++l) {
-    std::vector<int> slot_of_panel;
-    int64_t off = 0;
+    if (l <= P.dag_level0) off = 0;
     std::vector<std::pair<int, int>> diag_slots;  // panel, slot
     for (int ti : trsm_by_level[l]) {
       Tile& tl = P.tiles[ti];
@@ -549,6 +767,9 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
     if (off > 0x7fffffff) throw std::invalid_argument("diagonal scratch too large");
     P.diag_scratch = std::max(P.diag_scratch, off);
   }
+  std::vector<char> is_factor(P.tiles.size(), 0);
+  for (int l = 0; l < P.nlevel; ++l)
+    for (int ti : trsm_by_level[l]) is_factor[ti] = 1;
   // each level's factor list grouped by panel width class (one launch per class)
   P.lvl_trsm_cls.assign((size_t)4 * P.nlevel, 0);
   auto width_class = [&](int ti) {
@@ -572,6 +793,8 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
   }
   P.ftiles.reserve(P.trsm_tiles.size());
   for (int ti : P.trsm_tiles) P.ftiles.push_back(P.tiles[ti]);
+  if (P.dag_level0 < P.nlevel)
+    build_dag(P, task_src, tile_level, tile_flops, is_factor, params.dag);
   for (const Panel& pn : P.panels) {
     const double w = pn.ncol, m = pn.nrow - pn.ncol;
     P.factor_flops += w * w * w / 3.0 + m * w * w;
@@ -668,9 +891,15 @@ int swc_plan_sizes(const swc_plan* p, int64_t* out, double* flops) {
   out[10] = P.diag_scratch;
   out[11] = P.nslevel;
   out[12] = (int64_t)P.fused.size();
+  out[13] = (int64_t)P.dag_items.size();
+  out[14] = P.dag_level0;
+  out[15] = (int64_t)P.dag_deps.size();
   if (flops) {
     flops[0] = P.update_flops;
     flops[1] = P.factor_flops;
+    flops[2] = P.dag_flops;
+    flops[3] = P.dag_sim_us;
+    flops[4] = P.lvl_sim_us;
   }
   return SWC_OK;
 }
@@ -697,6 +926,8 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 18: return cp(P.lvl_fused);
     case 19: return cp(P.lvl_tiles_fused);
     case 20: return cp(P.lvl_tiles_narrow);
+    case 21: return cp(P.dag_items);
+    case 22: return cp(P.dag_deps);
     case 10: return cp(P.fblocks);
     case 11: return cp(P.lvl_fblk);
     case 12: return cp(P.gcols);

@@ -33,9 +33,12 @@ struct PlanParams {
                                // total inner dimension <= narrow_max_k
   int fuse_narrow = 1;         // 1: a level's panels whose tiles are all narrow are updated and
                                // factored together, one wave per panel from a device worklist
+  int dag = 0;                 // levels after the last one with narrow or fused tiles run as one
+                               // dependency-driven launch: 0 off, 1 in the order of a list
+                               // schedule simulated on the host, 2 in level order
 };
-// entries of the int[] form {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow}
-constexpr int kPlanParams = 5;
+// entries of the int[] form {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag}
+constexpr int kPlanParams = 6;
 // Checks and copies the int[] form; returns false if a value is out of range.
 bool plan_params_from(const int* v, int count, PlanParams& out);
 
@@ -101,6 +104,24 @@ struct FusedPanel {
   int ntile;
 };
 
+// One work item of the dependency-driven factor kernel: the updates one level
+// applies to one tile and, when that level completes the tile's panel, the
+// factorization of the tile's rows.  Items are claimed in array order, which is
+// a topological order: every item an item waits for comes before it.
+struct DagItem {
+  int tile;        // index into tiles
+  int fclass;      // -1: updates only; 0: factor, panel <= 16 columns; 1: 17..64 columns
+  int prev;        // the item that last wrote this tile (-1: written before the launch)
+  int diag;        // a factoring tile below the diagonal: its panel's diagonal-tile item
+  int dep0, dep1;  // [dep0, dep1) into dag_deps: panels the updates read
+};
+
+// Wait until `count` tiles of `panel` are factored (all of them: the panel is final).
+struct DagDep {
+  int panel;
+  int count;
+};
+
 static_assert(sizeof(Panel) == 48, "Panel layout is part of the plan export");
 static_assert(sizeof(Tile) == 48, "Tile layout is part of the plan export");
 static_assert(sizeof(Task) == 56, "Task layout is part of the plan export");
@@ -146,7 +167,16 @@ struct Plan {
   std::vector<int> lvl_fblk;        // nslevel+1
   std::vector<int> fblk_ptr;        // nsp+1: blocks of sp_order[i]
   int max_lvl_fblk = 0;             // most row blocks in one level
-  int64_t diag_scratch = 0;         // doubles of diagonal-block copies one level needs
+  int64_t diag_scratch = 0;         // doubles of diagonal-block copies one level needs (the
+                                    // levels of the dependency-driven launch hold theirs together)
+  // dependency-driven factor launch over levels dag_level0 .. nlevel-1 (none when
+  // dag_items is empty)
+  int dag_level0 = 0;
+  std::vector<DagItem> dag_items;
+  std::vector<DagDep> dag_deps;
+  double dag_flops = 0;             // useful flops of its updates and factorizations
+  double dag_sim_us = 0;            // makespan of the simulated list schedule (cost model)
+  double lvl_sim_us = 0;            // the same levels as per-level launches, same cost model
   int64_t vsize = 0;                // sum over supernodes of (rows - width)
   double update_flops = 0, factor_flops = 0;
   // algorithmic flops of each kernel class: lower-triangle update products (tile

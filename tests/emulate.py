@@ -15,93 +15,187 @@ def _bits(m):
     return np.array([i for i in range(64) if (m >> i) & 1], dtype=np.int64)
 
 
+def _update_tile(plan, Ls, L, ti, xsize, scratch):
+    """k_tile_update / tile_update_body: every task of tile ti, then the copy of an
+    updated diagonal block into its scratch slot (scratch: slot -> (panel, block))."""
+    panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
+    tl = tiles[ti]
+    pn = panels[tl["panel"]]
+    nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
+    assert 1 <= nr <= 64 and 1 <= nc <= 64
+    base = int(pn["vbase"]) + int(tl["r0"])
+    _chk(base, base + (nc - 1) * ld + nr, xsize, "tile")
+    ro = int(pn["rows_off"]) + int(tl["r0"])
+    _chk(ro, ro + nr, len(Ls), "tile rows")
+    trow = Ls[ro:ro + nr]
+    idx = base + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]
+    acc = L[idx].copy()
+    for t in range(tl["task_begin"], tl["task_end"]):
+        tk = tasks[t]
+        ra, rb, ca, cb = int(tk["ra"]), int(tk["rb"]), int(tk["ca"]), int(tk["cb"])
+        k, sld, sb = int(tk["k"]), int(tk["ld"]), int(tk["src_base"])
+        roff = int(tk["rows_off"])
+        assert ra < rb and ca < cb and rb - ra <= 64 and cb - ca <= 64
+        # landing positions: the e-th set bit receives source row ra+e / ca+e
+        pr = _bits(int(tk["rmask"]))
+        pc = _bits(int(tk["cmask"]))
+        assert len(pr) == rb - ra and len(pc) == cb - ca, "mask/row count mismatch"
+        assert pr.max() < nr and pc.max() < nc, "landing outside the tile"
+        _chk(roff + min(ra, ca), roff + max(rb, cb), len(Ls), "task rows")
+        assert np.array_equal(trow[pr], Ls[roff + ra:roff + rb]), "row mask"
+        assert np.array_equal(int(pn["c0"]) + pc, Ls[roff + ca:roff + cb]), "column mask"
+        # every buffer offset of a loaded element stays inside the
+        # source's k columns (row < ld), and those inside L
+        assert max(rb, cb) <= sld
+        _chk(sb, sb + (k - 1) * sld + max(rb, cb), xsize, "task src")
+        A = L[sb + np.arange(k)[None, :] * sld + np.arange(ra, rb)[:, None]]
+        B = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
+        acc[np.ix_(pr, pc)] -= A @ B.T
+    keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["r0"] == 0 else \
+        np.ones((nr, nc), bool)
+    L[idx[keep]] = acc[keep]
+    if tl["slot"] >= 0 and tl["kind"] == 0:
+        _chk(int(tl["slot"]), int(tl["slot"]) + nc * nc, plan["diag_scratch"], "diag slot")
+        scratch[int(tl["slot"])] = (int(tl["panel"]), np.tril(acc[:nc, :nc]).copy())
+
+
+def _factor_tile(plan, L, ti, xsize, scratch):
+    """k_panel_factor_regs / k_panel_factor_blk on tile ti; returns info (0 or the
+    failing global column + 1) for a diagonal tile."""
+    panels, tiles = plan["panels"], plan["tiles"]
+    tl = tiles[ti]
+    pn = panels[tl["panel"]]
+    nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
+    owner, blk = scratch[int(tl["slot"])]  # the launch-invariant copy
+    assert owner == int(tl["panel"]), "diagonal scratch slot holds another panel's block"
+    a11 = blk.copy()
+    first = nc if tl["kind"] == 0 else 0
+    base = int(pn["vbase"]) + int(tl["r0"])
+    if nc > 16:
+        bad = panel_factor_blk(L, base, ld, a11, nc, nr, first, xsize)
+    else:
+        bad = _panel_factor_scalar(L, base, ld, a11, nc, nr, first, xsize)
+    return int(pn["c0"]) + bad if bad and tl["kind"] == 0 else 0
+
+
+def _factor_level(plan, Ls, L, lvl, xsize, scratch):
+    panels, tiles = plan["panels"], plan["tiles"]
+    info = 0
+    t0, t1 = plan["lvl_tiles"][lvl], plan["lvl_tiles"][lvl + 1]
+    s0, s1 = plan["lvl_trsm"][lvl], plan["lvl_trsm"][lvl + 1]
+    for ti in range(t0, t1):
+        _update_tile(plan, Ls, L, ti, xsize, scratch)
+    # fused narrow panels: one wave updates (above) and factors each panel,
+    # its tiles contiguous with the diagonal tile first, none in the factor lists
+    if "fused" in plan:
+        in_lists = set(int(t) for t in plan["trsm_tiles"][s0:s1])
+        for it in plan["fused"][plan["lvl_fused"][lvl]:plan["lvl_fused"][lvl + 1]]:
+            t0f, ntf = int(it["tile0"]), int(it["ntile"])
+            assert t0 <= t0f and t0f + ntf <= t0 + plan["lvl_tiles_fused"][lvl]
+            d = tiles[t0f]
+            pn = panels[d["panel"]]
+            nc, ld = int(pn["ncol"]), int(pn["ld"])
+            assert d["kind"] == 0 and d["r0"] == 0 and nc <= 16 and int(pn["sw"]) == nc
+            didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
+            a11 = np.tril(L[didx])
+            for k in range(ntf):
+                tl = tiles[t0f + k]
+                assert tl["panel"] == d["panel"] and t0f + k not in in_lists
+                assert k == 0 or tl["r0"] > tiles[t0f + k - 1]["r0"]
+                first = nc if k == 0 else 0
+                base = int(pn["vbase"]) + int(tl["r0"])
+                assert (np.arange(64)[:, None] * 17 + np.arange(16)[None, :]).max() < 64 * 17
+                bad = _panel_factor_scalar(L, base, ld, a11.copy(), nc, int(tl["nr"]), first, xsize)
+                if bad and k == 0:
+                    info = info or int(pn["c0"]) + bad
+    # panel factor kernel: every listed tile recomputes L11 of its panel
+    for ti in plan["trsm_tiles"][s0:s1]:
+        info = info or _factor_tile(plan, L, int(ti), xsize, scratch)
+    return info
+
+
 def factor(plan, Ls, xsize, amap, vals):
     L = np.zeros(xsize)
     _chk(int(amap.min()), int(amap.max()) + 1, xsize, "amap")
     L[amap] = vals
-    panels, tiles, tasks = plan["panels"], plan["tiles"], plan["tasks"]
     info = 0
     for lvl in range(plan["nlevel"]):
-        scratch = {}
-        t0, t1 = plan["lvl_tiles"][lvl], plan["lvl_tiles"][lvl + 1]
-        s0, s1 = plan["lvl_trsm"][lvl], plan["lvl_trsm"][lvl + 1]
-        for ti in range(t0, t1):
-            tl = tiles[ti]
-            pn = panels[tl["panel"]]
-            nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
-            assert 1 <= nr <= 64 and 1 <= nc <= 64
-            base = int(pn["vbase"]) + int(tl["r0"])
-            _chk(base, base + (nc - 1) * ld + nr, xsize, "tile")
-            ro = int(pn["rows_off"]) + int(tl["r0"])
-            _chk(ro, ro + nr, len(Ls), "tile rows")
-            trow = Ls[ro:ro + nr]
-            idx = base + np.arange(nc)[None, :] * ld + np.arange(nr)[:, None]
-            acc = L[idx].copy()
-            for t in range(tl["task_begin"], tl["task_end"]):
-                tk = tasks[t]
-                ra, rb, ca, cb = int(tk["ra"]), int(tk["rb"]), int(tk["ca"]), int(tk["cb"])
-                k, sld, sb = int(tk["k"]), int(tk["ld"]), int(tk["src_base"])
-                roff = int(tk["rows_off"])
-                assert ra < rb and ca < cb and rb - ra <= 64 and cb - ca <= 64
-                # landing positions: the e-th set bit receives source row ra+e / ca+e
-                pr = _bits(int(tk["rmask"]))
-                pc = _bits(int(tk["cmask"]))
-                assert len(pr) == rb - ra and len(pc) == cb - ca, "mask/row count mismatch"
-                assert pr.max() < nr and pc.max() < nc, "landing outside the tile"
-                _chk(roff + min(ra, ca), roff + max(rb, cb), len(Ls), "task rows")
-                assert np.array_equal(trow[pr], Ls[roff + ra:roff + rb]), "row mask"
-                assert np.array_equal(int(pn["c0"]) + pc, Ls[roff + ca:roff + cb]), "column mask"
-                # every buffer offset of a loaded element stays inside the
-                # source's k columns (row < ld), and those inside L
-                assert max(rb, cb) <= sld
-                _chk(sb, sb + (k - 1) * sld + max(rb, cb), xsize, "task src")
-                A = L[sb + np.arange(k)[None, :] * sld + np.arange(ra, rb)[:, None]]
-                B = L[sb + np.arange(k)[None, :] * sld + np.arange(ca, cb)[:, None]]
-                acc[np.ix_(pr, pc)] -= A @ B.T
-            keep = np.arange(nr)[:, None] >= np.arange(nc)[None, :] if tl["r0"] == 0 else \
-                np.ones((nr, nc), bool)
-            L[idx[keep]] = acc[keep]
-            if tl["slot"] >= 0 and tl["kind"] == 0:
-                _chk(int(tl["slot"]), int(tl["slot"]) + nc * nc, plan["diag_scratch"], "diag slot")
-                scratch[int(tl["slot"])] = np.tril(acc[:nc, :nc]).copy()
-        # fused narrow panels: one wave updates (above) and factors each panel,
-        # its tiles contiguous with the diagonal tile first, none in the factor lists
-        if "fused" in plan:
-            in_lists = set(int(t) for t in plan["trsm_tiles"][s0:s1])
-            for it in plan["fused"][plan["lvl_fused"][lvl]:plan["lvl_fused"][lvl + 1]]:
-                t0f, ntf = int(it["tile0"]), int(it["ntile"])
-                assert t0 <= t0f and t0f + ntf <= t0 + plan["lvl_tiles_fused"][lvl]
-                d = tiles[t0f]
-                pn = panels[d["panel"]]
-                nc, ld = int(pn["ncol"]), int(pn["ld"])
-                assert d["kind"] == 0 and d["r0"] == 0 and nc <= 16 and int(pn["sw"]) == nc
-                didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
-                a11 = np.tril(L[didx])
-                for k in range(ntf):
-                    tl = tiles[t0f + k]
-                    assert tl["panel"] == d["panel"] and t0f + k not in in_lists
-                    assert k == 0 or tl["r0"] > tiles[t0f + k - 1]["r0"]
-                    first = nc if k == 0 else 0
-                    base = int(pn["vbase"]) + int(tl["r0"])
-                    assert (np.arange(64)[:, None] * 17 + np.arange(16)[None, :]).max() < 64 * 17
-                    bad = _panel_factor_scalar(L, base, ld, a11.copy(), nc, int(tl["nr"]), first, xsize)
-                    if bad and k == 0:
-                        info = info or int(pn["c0"]) + bad
-        # panel factor kernel: every listed tile recomputes L11 of its panel
-        for ti in plan["trsm_tiles"][s0:s1]:
-            tl = tiles[ti]
-            pn = panels[tl["panel"]]
-            nr, nc, ld = int(tl["nr"]), int(pn["ncol"]), int(pn["ld"])
-            didx = int(pn["vbase"]) + np.arange(nc)[None, :] * ld + np.arange(nc)[:, None]
-            a11 = scratch[int(tl["slot"])].copy()  # the launch-invariant copy
-            first = nc if tl["kind"] == 0 else 0
-            base = int(pn["vbase"]) + int(tl["r0"])
-            if nc > 16:
-                bad = panel_factor_blk(L, base, ld, a11, nc, nr, first, xsize)
+        info = info or _factor_level(plan, Ls, L, lvl, xsize, {})
+    return L, info
+
+
+def factor_dag(plan, Ls, xsize, amap, vals, seed=0, workers=8):
+    """The per-level launches up to dag_level0, then k_dag_factor: `workers`
+    workgroups claim the work items in array order and advance, one step at a
+    time in a random interleaving, through wait -> update -> publish -> (wait for
+    the diagonal tile) -> factor -> count the panel's factored tile.  A wait may
+    only name items claimed before (checked), and the run must never stall; the
+    diagonal scratch slots of the launch's levels are shared by all of it, so a
+    slot overwritten while another tile still needs it is caught."""
+    rng = np.random.default_rng(seed)
+    L = np.zeros(xsize)
+    L[amap] = vals
+    info = 0
+    l0 = int(plan["dag_level0"])
+    items, deps, tiles = plan["dag_items"], plan["dag_deps"], plan["tiles"]
+    if len(items) == 0:
+        l0 = plan["nlevel"]
+    for lvl in range(l0):
+        info = info or _factor_level(plan, Ls, L, lvl, xsize, {})
+    if len(items) == 0:
+        return L, info
+    assert int(items[0]["tile"]) >= plan["lvl_tiles"][l0]
+    scratch = {}
+    n = len(items)
+    state = np.zeros(n, np.int64)
+    pdone = np.zeros(len(plan["panels"]), np.int64)
+    ticket = 0
+    work = [None] * workers  # [item, phase]
+    done = 0
+    while done < n:
+        movable = []
+        for w in range(workers):
+            if work[w] is None:
+                if ticket < n:
+                    movable.append(w)
+                continue
+            k, ph = work[w]
+            d = items[k]
+            if ph == 0:  # wait for the tile's previous writer and the source panels
+                ok = d["prev"] < 0 or state[d["prev"]] >= 1
+                for e in range(d["dep0"], d["dep1"]):
+                    ok = ok and pdone[deps[e]["panel"]] >= deps[e]["count"]
+                if ok:
+                    movable.append(w)
+            elif ph == 1:  # wait for the diagonal tile's updates
+                if d["diag"] < 0 or state[d["diag"]] >= 1:
+                    movable.append(w)
+        assert movable, "the dependency-driven launch stalls"
+        w = int(rng.choice(movable))
+        if work[w] is None:
+            work[w] = [ticket, 0]
+            d = items[ticket]
+            assert d["prev"] < ticket and d["diag"] < ticket
+            ticket += 1
+            continue
+        k, ph = work[w]
+        d = items[k]
+        ti = int(d["tile"])
+        if ph == 0:
+            _update_tile(plan, Ls, L, ti, xsize, scratch)
+            if d["fclass"] < 0 or tiles[ti]["kind"] == 0:
+                state[k] = 1
+            if d["fclass"] < 0:
+                work[w] = None
+                done += 1
             else:
-                bad = _panel_factor_scalar(L, base, ld, a11, nc, nr, first, xsize)
-            if bad and tl["kind"] == 0:
-                info = info or int(pn["c0"]) + bad
+                assert d["fclass"] == (0 if tiles[ti]["ncol"] <= 16 else 1)
+                work[w][1] = 1
+        else:
+            info = info or _factor_tile(plan, L, ti, xsize, scratch)
+            pdone[tiles[ti]["panel"]] += 1
+            work[w] = None
+            done += 1
     return L, info
 
 

@@ -341,3 +341,40 @@ def test_fused_narrow_panels_match_two_kernel_path(name, make, mode):
     assert np.array_equal(out[0], out[1])
     if mode == 0:
         assert relF(out[1], Lo) <= 1e-12
+
+
+DAG_CASES = [SMALL[2], SMALL[3], SMALL[4], SMALL[6], ("lap3d_var_32", lambda: P.laplacian3d_var(32))]
+
+
+@pytest.mark.parametrize("name,make", DAG_CASES, ids=[c[0] for c in DAG_CASES])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_dependency_driven_launch_matches_level_launches(name, make, mode):
+    """The levels after the narrow-panel ones as one dependency-driven launch (in
+    list-schedule order, in level order, and with every level in it) give bitwise
+    the factor of the per-level launches, on the first (captured) and later
+    (replayed) factorizations, and the oracle's within 1e-12 in FP64."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    ref = S.Factor(b, params=(8, 128, 1024, 256, 1, 0))
+    ref.set_precision(mode)
+    assert ref.factor(prob.values) == 0
+    Lref = ref.L_values()
+    if mode == 0:
+        assert relF(Lref, Lo) <= 1e-12
+    for params in ((8, 128, 1024, 256, 1, 1), (8, 128, 1024, 256, 1, 2),
+                   (8, 128, 1 << 30, 256, 0, 1)):
+        print(f"[dag] {name} mode {mode} params {params}", flush=True)
+        f = S.Factor(b, params=params)
+        f.set_precision(mode)
+        for rep in range(3):
+            assert f.factor(prob.values) == 0
+            assert np.array_equal(f.L_values(), Lref), (params, rep)
+
+
+def test_dependency_driven_launch_reports_non_positive_pivot():
+    prob = P.laplacian2d_var(20, 5, shift=-2.0)  # strongly indefinite
+    b, a2x, Lo, info_o = setup(prob)
+    assert info_o > 0
+    for params in ((8, 128, 1 << 30, 256, 0, 1), (8, 128, 1 << 30, 256, 0, 0)):
+        f = S.Factor(b, params=params)
+        assert f.factor(prob.values) == info_o

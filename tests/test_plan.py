@@ -220,3 +220,84 @@ def test_hlevel_schedule_plan(name):
         prev = [int(s) for q in range(lp[k - 1], lp[k]) for s in part[pr[q]:pr[q + 1]]]
         cur = [int(s) for q in range(lp[k], lp[k + 1]) for s in part[pr[q]:pr[q + 1]]]
         assert min(first[s] for s in cur) > max(last[s] for s in prev)
+
+
+def _dag_structure_checks(plan):
+    """The work items of the dependency-driven launch cover each tile entry of its
+    levels once; prev is the tile's entry at its previous level; a dependency on
+    a panel waits for all of its factoring tiles; the diagonal scratch slots of
+    the launch's panels do not overlap (they are all in flight together)."""
+    items, deps, tiles = plan["dag_items"], plan["dag_deps"], plan["tiles"]
+    l0 = plan["dag_level0"]
+    t0 = plan["lvl_tiles"][l0]
+    assert sorted(items["tile"].tolist()) == list(range(t0, len(tiles)))
+    level = np.zeros(len(tiles), np.int64)
+    for lvl in range(plan["nlevel"]):
+        level[plan["lvl_tiles"][lvl]:plan["lvl_tiles"][lvl + 1]] = lvl
+    factoring = set(int(t) for t in plan["trsm_tiles"])
+    nfac = np.zeros(len(plan["panels"]), np.int64)
+    for t in range(t0, len(tiles)):
+        if t in factoring:
+            nfac[tiles[t]["panel"]] += 1
+    last = {}
+    for k, d in enumerate(items):
+        tl = tiles[d["tile"]]
+        key = (int(tl["panel"]), int(tl["r0"]))
+        if d["prev"] >= 0:
+            p = items[d["prev"]]
+            assert d["prev"] < k and (int(tiles[p["tile"]]["panel"]), int(tiles[p["tile"]]["r0"])) == key
+            assert level[p["tile"]] < level[d["tile"]]
+        assert last.get(key, -1) == d["prev"]
+        last[key] = k
+        assert (d["fclass"] >= 0) == (int(d["tile"]) in factoring)
+        for e in range(d["dep0"], d["dep1"]):
+            assert deps[e]["count"] == nfac[deps[e]["panel"]] > 0
+    spans = sorted((int(tiles[t]["slot"]), int(tiles[t]["ncol"]) ** 2)
+                   for t in range(t0, len(tiles)) if t in factoring and tiles[t]["kind"] == 0)
+    for (a, na), (b2, _) in zip(spans, spans[1:]):
+        assert a + na <= b2
+    assert spans[-1][0] + spans[-1][1] <= plan["diag_scratch"]
+
+
+@pytest.mark.parametrize("params", [(8, 128, 1024, 256, 1, 1), (8, 128, 1024, 256, 1, 2),
+                                    (2, 128, 1 << 30, 256, 0, 1), (3, 64, 1, 256, 1, 1)])
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(12), lambda: P.dense_spd(300),
+                                  lambda: P.laplacian2d_var(37, 3), lambda: P.vector3d_var(6)])
+def test_dependency_driven_launch_emulated(params, make):
+    """The dependency-driven factor launch, emulated: workgroups claim items in
+    the plan's order and advance in random interleavings, each item waiting as the
+    kernel waits.  The result is bitwise the per-level launches' (the waits leave
+    no read of a value before it is final), no wait names a later item, and no
+    interleaving stalls."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    plan = api.build_plan(b, params=params)
+    assert len(plan["dag_items"]) > 0
+    _dag_structure_checks(plan)
+    amap = _amap(b)
+    Ll, info_l = E.factor(plan, b.s, b.xsize, amap, a2x)
+    assert info_l == 0 and relF(Ll, Lo) < 1e-12
+    for seed, workers in ((0, 3), (1, 16)):
+        Ld, info_d = E.factor_dag(plan, b.s, b.xsize, amap, a2x, seed=seed, workers=workers)
+        assert info_d == info_l
+        assert np.array_equal(Ld, Ll)
+
+
+def test_dependency_emulation_catches_a_missing_wait():
+    """The emulation has the power to see a missing wait: with the sources'
+    dependencies dropped from the plan, some interleaving reads a panel before it
+    is final and the factor differs."""
+    prob = P.laplacian3d_var(10)
+    b, a2x, Lo, _ = setup(prob)
+    plan = api.build_plan(b, params=(8, 128, 1 << 30, 256, 0, 2))
+    amap = _amap(b)
+    Ll, _ = E.factor(plan, b.s, b.xsize, amap, a2x)
+    broken = dict(plan)
+    broken["dag_items"] = plan["dag_items"].copy()
+    broken["dag_items"]["dep1"] = broken["dag_items"]["dep0"]
+    differs = False
+    with np.errstate(all="ignore"):  # values read too early may overflow
+        for seed in range(4):
+            Ld, _ = E.factor_dag(broken, b.s, b.xsize, amap, a2x, seed=seed, workers=16)
+            differs = differs or not np.array_equal(Ld, Ll)
+    assert differs
