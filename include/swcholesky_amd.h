@@ -137,13 +137,16 @@ int swc_upload_symbolic(swc_handle* h, const swc_symbolic* sym);
 int swc_set_perm(swc_handle* h, const int* perm);
 
 /* Shape parameters of the plan built by the next structure upload (swc_upload,
- * swc_upload_symbolic), as {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow}
- * (count <= 5, leading entries; NULL restores the defaults {8, 128, 1024, 256, 1}):
- * rlb panels per block of a wide supernode's right-looking steps, spw = 64 or 128
- * columns per solve panel, the one-wave tile kernel for levels with at least
+ * swc_upload_symbolic), as {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow,
+ * dag} (count <= 6, leading entries; NULL restores the defaults {8, 128, 1024, 256,
+ * 1, 0}): rlb panels per block of a wide supernode's right-looking steps, spw = 64
+ * or 128 columns per solve panel, the one-wave tile kernel for levels with at least
  * narrow_min_tiles tiles of panels <= 16 columns whose inner dimensions sum to
- * <= narrow_max_k, and with fuse_narrow = 1 such panels (all tiles narrow) updated
- * and factored by one wave each in one persistent launch per level.  No reference counterpart (the reference's Sunway BLAS fixes
+ * <= narrow_max_k, with fuse_narrow = 1 such panels (all tiles narrow) updated
+ * and factored by one wave each in one persistent launch per level, and with
+ * dag = 1 (list-schedule order) or 2 (level order) the levels after those as one
+ * dependency-driven persistent launch (bitwise the same factor; a bounded wait
+ * that runs out makes swc_factor fail with SWC_EDEVICE).  No reference counterpart (the reference's Sunway BLAS fixes
  * its blocking, BLAS-sw/dpotrf_cluster_master.c:11-63); the values come from the
  * autotuner's table (swcholesky_amd/tuning.json). */
 int swc_set_plan_params(swc_handle* h, const int* params, int count);

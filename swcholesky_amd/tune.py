@@ -3,7 +3,7 @@
     python -m swcholesky_amd.tune [--workloads lap3d64,vec48,gmrf256,batch]
                                   [--reps 3] [--out swcholesky_amd/tuning.json]
 
-For each workload it searches {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow}
+For each workload it searches {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag}
 (swc_set_plan_params) by coordinate descent from the library defaults: one
 parameter at a time over its candidate values, keeping the best, until a full
 pass changes nothing.  The objective is the measured FP64 time to a solution,
@@ -34,6 +34,7 @@ CANDIDATES = {
     "narrow_min_tiles": [256, 1024, 4096, 1 << 30],
     "narrow_max_k": [128, 256, 512],
     "fuse_narrow": [0, 1],
+    "dag": [0, 1, 2],
 }
 
 

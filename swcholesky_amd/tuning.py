@@ -3,7 +3,7 @@
 The table (``tuning.json`` next to this file) is written by ``python -m
 swcholesky_amd.tune`` on an MI355X: for each benchmark workload it records the
 structure's class (order n, widest supernode) and the parameters
-{rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow} that gave the shortest measured
+{rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag} that gave the shortest measured
 factor + solve time.  A structure takes the entry of the nearest class in
 (log2 n, log2 widest supernode); with no table it keeps the library defaults.
 """
@@ -15,8 +15,8 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-NAMES = ("rlb", "spw", "narrow_min_tiles", "narrow_max_k", "fuse_narrow")
-DEFAULTS = (8, 128, 1024, 256, 1)
+NAMES = ("rlb", "spw", "narrow_min_tiles", "narrow_max_k", "fuse_narrow", "dag")
+DEFAULTS = (8, 128, 1024, 256, 1, 0)
 
 _table = None
 
@@ -46,7 +46,7 @@ def _distance(a, b):
 
 
 def params_for(bcsc, entries=None):
-    """(rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow) for a symbolic structure."""
+    """(rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag) for a symbolic structure."""
     entries = load_table() if entries is None else entries
     if not entries:
         return DEFAULTS

@@ -180,7 +180,7 @@ def test_plan_shape_parameters_are_checked_and_looked_up():
             api.build_plan(b, params=bad)
     entries = [{"class": {"n": 262144, "max_width": 6593}, "params": dict(zip(tuning.NAMES, (4, 64, 256, 128)))},
                {"class": {"n": 1369, "max_width": 40}, "params": dict(zip(tuning.NAMES, (16, 128, 4096, 512)))}]
-    assert tuning.params_for(b, entries) == (16, 128, 4096, 512, 1)  # missing keys: defaults
+    assert tuning.params_for(b, entries) == (16, 128, 4096, 512, 1, 0)  # missing keys: defaults
     assert tuning.params_for(b, []) == tuning.DEFAULTS
 
 
