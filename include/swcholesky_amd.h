@@ -263,7 +263,8 @@ int swc_get_timings(const swc_handle* h, double* out);
  * level->solve-panel offsets, 17 fused narrow panels (int pairs: first tile, tile
  * count), 18 level->fused-panel offsets, 19 per level: tiles of fused panels, 20
  * per level: one-wave tiles, 21 dependency-driven work items (DagItem), 22 their
- * dependencies (int pairs: panel, tile count)) as the packed structs and arrays of
+ * dependencies (int pairs: panel, tile count), 23 per level: tiles of panels the
+ * level does not factor, last in the level) as the packed structs and arrays of
  * swcholesky_amd/csrc/plan.h.  With prunePtr NULL the descendant sets are derived
  * from the row lists (supernode s depends on every d whose rows below its own
  * columns meet s's columns) and only the solve part of the plan is built, as

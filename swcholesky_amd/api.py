@@ -395,11 +395,13 @@ def build_plan(bcsc: BCSC, derive_sets=False, params=None, schedule=None):
                 (15, np.int32, nsp), (16, np.int32, nslevel + 1),
                 (17, np.dtype([("tile0", np.int32), ("ntile", np.int32)]), nfused),
                 (18, np.int32, nlevel + 1), (19, np.int32, nlevel), (20, np.int32, nlevel),
-                (21, DAG_DT, ndag), (22, np.dtype([("panel", np.int32), ("count", np.int32)]), ndep)]
+                (21, DAG_DT, ndag), (22, np.dtype([("panel", np.int32), ("count", np.int32)]), ndep),
+                (23, np.int32, nlevel)]
         names = ["panels", "tiles", "tasks", "lvl_tiles", "lvl_trsm", "panel_order",
                  "lvl_panels", "gptr", "gidx", "trsm_tiles", "fblocks", "lvl_fblk",
                  "gcols", "lvl_gcol", "spanels", "sp_order", "lvl_sp", "fused", "lvl_fused",
-                 "lvl_tiles_fused", "lvl_tiles_narrow", "dag_items", "dag_deps"]
+                 "lvl_tiles_fused", "lvl_tiles_narrow", "dag_items", "dag_deps",
+                 "lvl_tiles_deferred"]
         out = {}
         for (which, dt, cnt), name in zip(spec, names):
             a = np.zeros(cnt, dtype=dt)

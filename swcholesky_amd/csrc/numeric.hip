@@ -903,6 +903,27 @@ k_dag_factor(double* __restrict__ L, const Tile* __restrict__ tiles,
   }
 }
 
+// A level's panel factorization (panels of 17..64 columns, workgroups
+// [0, nfac): k_panel_factor_blk) in the same launch as the level's updates of
+// tiles whose panels it does not factor (the remaining workgroups:
+// k_tile_update), so that those updates, which nothing of this level waits for,
+// overlap the factorization instead of delaying it.  Same bodies, same results.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_factor_and_update(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0, int nfac,
+                    const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
+                    double* __restrict__ dscratch, int* __restrict__ info_out, float* Lf_factor,
+                    const float* __restrict__ Lf_update) {
+  __shared__ double lds[kDagLds];
+  if ((int)blockIdx.x < nfac) {
+    const Tile tl = ftiles[list0 + blockIdx.x];
+    panel_factor_blk_body(L, tl, dscratch, info_out, Lf_factor, lds, threadIdx.x);
+  } else {
+    const Tile tl = tiles[tile0 + (blockIdx.x - nfac)];
+    tile_update_body<MODE>(L, tl, tasks, dscratch, Lf_update, lds, threadIdx.x);
+  }
+}
+
 // CSC export of the factor (the layout of the reference's bcsc2csc,
 // common/Util.h:311-338): column j's entries are its supernode's rows from its
 // own diagonal down.  col[j] = {value offset of L(j, j), offset in Ls of row j,
@@ -2195,25 +2216,39 @@ void enqueue_factor(swc_handle* h, bool profile) {
       hipLaunchKernelGGL(kern, dim3(ntn), dim3(64), 0, st, h->dL, (const Tile*)h->dTiles,
                          (const Task*)h->dTasks, t0 + ntf, h->dDiag, (const float*)h->dLf);
     }
-    if (nt > ntf + ntn) {
+    // tiles of the panels this level factors, then (last in the level's list) the
+    // others, which run beside the panel factorization unless profiling times the
+    // update launches on their own
+    const int ntd = profile ? 0 : P.lvl_tiles_deferred[l];
+    const int tc0 = t0 + ntf + ntn, ntc = nt - ntf - ntn - ntd;
+    if (ntc > 0) {
       auto kern = h->precision == 0 ? k_tile_update<0> : (h->precision == 1 ? k_tile_update<1>
                                                                              : k_tile_update<2>);
-      hipLaunchKernelGGL(kern, dim3(nt - ntf - ntn), dim3(kThreads), 0, st, h->dL,
-                         (const Tile*)h->dTiles, (const Task*)h->dTasks, t0 + ntf + ntn, h->dDiag,
-                         (const float*)h->dLf);
+      hipLaunchKernelGGL(kern, dim3(ntc), dim3(kThreads), 0, st, h->dL, (const Tile*)h->dTiles,
+                         (const Task*)h->dTasks, tc0, h->dDiag, (const float*)h->dLf);
     }
     if (pe) (void)hipEventRecord(pe[1], st);
-    if (ns > 0) {
+    {
       // the level's panels by width class: <= 16, <= 32, <= 64 columns
       const int* cls = &P.lvl_trsm_cls[(size_t)4 * l];
       float* lf = h->fp32_mask ? h->dLf : nullptr;
       const Tile* ft = (const Tile*)h->dFtiles;
-      if (cls[1] > cls[0])
+      const int nblk = ns > 0 ? cls[3] - cls[1] : 0;
+      if (ns > 0 && cls[1] > cls[0])
         hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, ft,
                            cls[0], (const double*)h->dDiag, h->dInfo, lf);
-      if (cls[3] > cls[1])
-        hipLaunchKernelGGL(k_panel_factor_blk, dim3(cls[3] - cls[1]), dim3(256), 0, st, h->dL, ft,
+      if (ntd > 0) {
+        auto kern = h->precision == 0
+                        ? k_factor_and_update<0>
+                        : (h->precision == 1 ? k_factor_and_update<1> : k_factor_and_update<2>);
+        hipLaunchKernelGGL(kern, dim3(nblk + ntd), dim3(kThreads), 0, st, h->dL, ft,
+                           ns > 0 ? cls[1] : 0, nblk, (const Tile*)h->dTiles,
+                           (const Task*)h->dTasks, tc0 + ntc, h->dDiag, h->dInfo, lf,
+                           (const float*)h->dLf);
+      } else if (nblk > 0) {
+        hipLaunchKernelGGL(k_panel_factor_blk, dim3(nblk), dim3(256), 0, st, h->dL, ft,
                            cls[1], (const double*)h->dDiag, h->dInfo, lf);
+      }
     }
     if (pe) (void)hipEventRecord(pe[2], st);
   }

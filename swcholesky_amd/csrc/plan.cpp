@@ -662,6 +662,46 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       if (fused((int)i) || narrow((int)i)) last = std::max(last, ptiles[i].level);
     P.dag_level0 = last + 1;
   }
+  // Then, in levels where it pays, the tiles of the panels the level factors
+  // before the others (later panels of a wide supernode): only the former hold
+  // up the level's panel factorization, so the latter can run in the same
+  // launch as it.  It pays where the former are light (a right-looking step's
+  // next panel) and not where they weigh as much as the rest (a wide
+  // supernode's external updates): per level, with launch durations estimated
+  // as max(longest tile, total / 512 workgroups + half the longest tile) from
+  // each tile's update flops (3 us + 1.4 flops / 1e5 per us and workgroup) and
+  // ~20 us for the panel factorization (measured on MI355X), the level defers
+  // when critical + max(factor, deferred) beats all tiles + factor by 5%.
+  auto critical = [&](int i) { return ptiles[i].trsm || ptiles[i].t.kind == 0; };
+  std::vector<char> defer_level(P.nlevel, 0);
+  {
+    std::vector<double> tus(ptiles.size(), 3.0);
+    for (size_t i = 0; i < ptiles.size(); ++i)
+      for (const Task& t : ptasks[i])
+        tus[i] += 1.4 * 2.0 * (double)(t.rb - t.ra) * (t.cb - t.ca) * t.k / 1e5;
+    std::vector<double> cmax(P.nlevel, 0), csum(P.nlevel, 0), dmax(P.nlevel, 0), dsum(P.nlevel, 0);
+    std::vector<char> has_blk(P.nlevel, 0);
+    for (size_t i = 0; i < ptiles.size(); ++i) {
+      const int l = ptiles[i].level;
+      if (fused((int)i) || narrow((int)i)) continue;
+      if (critical((int)i)) {
+        cmax[l] = std::max(cmax[l], tus[i]);
+        csum[l] += tus[i];
+        if (P.panels[ptiles[i].t.panel].ncol > 16) has_blk[l] = 1;
+      } else {
+        dmax[l] = std::max(dmax[l], tus[i]);
+        dsum[l] += tus[i];
+      }
+    }
+    for (int l = 0; l < P.nlevel; ++l) {
+      if (dsum[l] == 0 || !has_blk[l]) continue;
+      auto span = [](double mx, double sum) { return std::max(mx, sum / 512 + 0.5 * mx); };
+      const double tc = span(cmax[l], csum[l]), td = span(dmax[l], dsum[l]);
+      const double tcd = span(std::max(cmax[l], dmax[l]), csum[l] + dsum[l]);
+      const double tf = 20.0;
+      defer_level[l] = tc + std::max(tf, td) < 0.95 * (tcd + tf);
+    }
+  }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (ptiles[a].level != ptiles[b].level) return ptiles[a].level < ptiles[b].level;
     if (fused(a) != fused(b)) return fused(a);
@@ -670,13 +710,18 @@ void build_plan(int n, int nsuper, const int* super, const int64_t* pi, const in
       return ptiles[a].t.r0 < ptiles[b].t.r0;
     }
     if (narrow(a) != narrow(b)) return narrow(a);
+    if (!narrow(a) && defer_level[ptiles[a].level] && critical(a) != critical(b))
+      return critical(a);
     return cost[a] > cost[b];
   });
   P.lvl_tiles_fused.assign(P.nlevel, 0);
   P.lvl_tiles_narrow.assign(P.nlevel, 0);
+  P.lvl_tiles_deferred.assign(P.nlevel, 0);
   for (size_t i = 0; i < ptiles.size(); ++i) {
     if (fused((int)i)) P.lvl_tiles_fused[ptiles[i].level]++;
     else if (is_narrow[i]) P.lvl_tiles_narrow[ptiles[i].level]++;
+    else if (defer_level[ptiles[i].level] && !critical((int)i))
+      P.lvl_tiles_deferred[ptiles[i].level]++;
   }
   P.lvl_fused.assign(P.nlevel + 1, 0);
   P.lvl_tiles.assign(P.nlevel + 1, 0);
@@ -928,6 +973,7 @@ int swc_plan_export(const swc_plan* p, int which, void* dst) {
     case 20: return cp(P.lvl_tiles_narrow);
     case 21: return cp(P.dag_items);
     case 22: return cp(P.dag_deps);
+    case 23: return cp(P.lvl_tiles_deferred);
     case 10: return cp(P.fblocks);
     case 11: return cp(P.lvl_fblk);
     case 12: return cp(P.gcols);

@@ -136,6 +136,9 @@ struct Plan {
   std::vector<int> lvl_tiles_fused;   // nlevel: tiles of fused narrow panels, first in their level
   std::vector<int> lvl_tiles_narrow;  // nlevel: one-wave tiles (panels <= 16 wide, short
                                       // total inner dimension), next in their level
+  std::vector<int> lvl_tiles_deferred;  // nlevel: tiles of panels the level does not factor
+                                        // (later panels of a wide supernode), last in their
+                                        // level: they run beside the level's panel factor
   std::vector<FusedPanel> fused;      // fused narrow panels, by level
   std::vector<int> lvl_fused;         // nlevel+1 offsets into fused
   std::vector<int> trsm_tiles;      // tiles of the panels each level completes (factor kernel)

@@ -378,3 +378,22 @@ def test_dependency_driven_launch_reports_non_positive_pivot():
     for params in ((8, 128, 1 << 30, 256, 0, 1), (8, 128, 1 << 30, 256, 0, 0)):
         f = S.Factor(b, params=params)
         assert f.factor(prob.values) == info_o
+
+
+@pytest.mark.parametrize("name,make", [("dense_2000", lambda: P.dense_spd(2000))],
+                         ids=["dense_2000"])
+def test_deferred_updates_beside_panel_factor_match_separate_launches(name, make):
+    """Levels that run their other panels' updates in the launch of their panel
+    factorization give bitwise the factor of separate update and factor launches
+    (what the per-launch timing mode runs)."""
+    prob = make()
+    b, a2x, Lo, _ = setup(prob)
+    assert S.api.build_plan(b)["lvl_tiles_deferred"].sum() > 0
+    out = []
+    for profile in (False, True):
+        f = S.Factor(b, params=None)
+        f.set_profiling(profile)
+        assert f.factor(prob.values) == 0
+        out.append(f.L_values())
+    assert np.array_equal(out[0], out[1])
+    assert relF(out[0], Lo) <= 1e-12

@@ -301,3 +301,26 @@ def test_dependency_emulation_catches_a_missing_wait():
             Ld, _ = E.factor_dag(broken, b.s, b.xsize, amap, a2x, seed=seed, workers=16)
             differs = differs or not np.array_equal(Ld, Ll)
     assert differs
+
+
+@pytest.mark.parametrize("make", [lambda: P.laplacian3d_var(64), lambda: P.dense_spd(2000)])
+def test_deferred_tiles_are_last_and_not_factored_in_their_level(make):
+    """Tiles a level runs beside its panel factorization (deferred) come last in
+    the level's list, and none of them belongs to a panel that level factors (so
+    nothing of the level reads what they write)."""
+    prob = make()
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    plan = api.build_plan(b)
+    tiles, lt, ld = plan["tiles"], plan["lvl_tiles"], plan["lvl_tiles_deferred"]
+    factoring = set(int(t) for t in plan["trsm_tiles"])
+    panel_level = np.zeros(len(plan["panels"]), np.int64)
+    for lvl in range(plan["nlevel"]):
+        panel_level[plan["panel_order"][plan["lvl_panels"][lvl]:plan["lvl_panels"][lvl + 1]]] = lvl
+    assert ld.sum() > 0  # these structures have right-looking steps that defer
+    for lvl in range(plan["nlevel"]):
+        d = int(ld[lvl])
+        fac_panels = set(int(tiles[t]["panel"]) for t in range(lt[lvl], lt[lvl + 1]) if t in factoring)
+        for t in range(lt[lvl + 1] - d, lt[lvl + 1]):
+            assert t not in factoring and tiles[t]["kind"] == 1
+            assert int(tiles[t]["panel"]) not in fac_panels
+            assert panel_level[tiles[t]["panel"]] > lvl
