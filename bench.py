@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="factor steps only (for counter runs): no refinement paths, no "
                          "per-launch event pass, no CPU baseline")
+    ap.add_argument("--per-launch", action="store_true",
+                    help="with --quick: factor in the per-launch timing mode (each level's "
+                         "updates in one launch of their own, as the roofline pass times "
+                         "them), so that counters per tile-update launch match it")
     ap.add_argument("--workload", default="lap3d",
                     choices=["lap3d", "vec3d", "batch", "gmrf", "schedules"],
                     help="lap3d: the headline 64^3 operator (default); vec3d: 48^3 x 3-dof "
@@ -142,6 +146,8 @@ def main():
     log(f"plan {stats}")
     dvals = torch.from_numpy(prob.values).cuda()
     torch.cuda.synchronize()
+    if args.quick and args.per_launch:
+        f.set_profiling(True)
 
     for i in range(args.warmup):
         info = f.factor(dvals)
