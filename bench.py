@@ -397,7 +397,8 @@ def bench_gmrf(args):
     f = S.Factor(b, params=args.plan)
     vals = torch.tensor(prob.values, device="cuda", requires_grad=True)
     rhs = torch.tensor(np.random.default_rng(4).standard_normal(prob.n), device="cuda")
-    for rep in range(2):
+    fw, bw = [], []
+    for rep in range(7):  # two warm-ups (graph captures, selected-inverse set-up), median of 5
         vals.grad = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -407,7 +408,11 @@ def bench_gmrf(args):
         out.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-    res = {"forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "loss": float(out.detach()),
+        if rep >= 2:
+            fw.append((t1 - t0) * 1e3)
+            bw.append((t2 - t1) * 1e3)
+    res = {"forward_ms": float(np.median(fw)), "backward_ms": float(np.median(bw)),
+           "backward_runs_ms": [round(v, 3) for v in bw], "loss": float(out.detach()),
            "n": prob.n, "flops_F": b.flops}
     log(f"gmrf: {res}")
     print(json.dumps({"metric": "gradient of logdet(Q) + b^T Q^-1 b (256^2 GMRF)",
