@@ -389,7 +389,9 @@ __device__ inline void panel_factor_regs_body(double* __restrict__ L, const Tile
       if (!ok && badcol == 0) badcol = j + 1;
       // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column by
       // 1/ajj); a non-positive pivot continues with 1 so no NaNs spread
-      const double rs = ok ? rsqrt(d) : 1.0;
+      double r = rsqrt(d);
+      asm volatile("" : "+v"(r));  // computed whatever d is: a select, not a branch
+      const double rs = ok ? r : 1.0;
       if (lane == j) y[j] = ok ? d * rs : 1.0;
       else if (lane > j) y[j] *= rs;
 #pragma unroll
@@ -527,7 +529,9 @@ k_narrow_fused(double* __restrict__ L, const Tile* __restrict__ tiles,
             const double d = bcast(y[j], j);
             const bool ok = d > 0.0;
             if (!ok && badcol == 0) badcol = j + 1;
-            const double rs = ok ? rsqrt(d) : 1.0;
+            double r = rsqrt(d);
+            asm volatile("" : "+v"(r));  // a select, not a branch
+            const double rs = ok ? r : 1.0;
             if (lane == j) y[j] = ok ? d * rs : 1.0;
             else if (lane > j) y[j] *= rs;
 #pragma unroll
@@ -616,7 +620,10 @@ __device__ inline d4 lds_block_update(const double* __restrict__ W, d4 acc, int 
 //      pivot chain does not wait for LDS;
 //  (2) the later block columns of the diagonal block (blocks on or below its
 //      diagonal) and of the tile rows take the rank-16 update on FP64 MFMA,
-//      the 16x16 output blocks spread over the four waves.
+//      the 16x16 output blocks spread over the four waves;
+//  (3) while waves 0-2 sweep block column J, wave 3 stores block column J-1,
+//      which is final, to L, so that only the last block column is stored
+//      after the loop.
 // Both apply each entry's updates in increasing column order, so the result is
 // bitwise the one of a column-by-column right-looking factorization.  Every
 // tile of a panel computes the same L11, so no tile waits for another; the
@@ -638,6 +645,7 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
   const int nxb = (nx + 15) >> 4;
   const double* gdiag = dscratch + tl.slot;  // updated diagonal block, ld nc
   double* gtile = L + tl.vbase + tl.r0;
+  float* ftile = Lf ? Lf + tl.vbase + tl.r0 : nullptr;
   {
     // wave w moves columns 16w..16w+15 of diagonal-block row `lane` and of tile
     // row `lane`; every lane loads from an address inside both buffers and then
@@ -661,21 +669,57 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
       }
     }
   }
+  // Final columns c0 + cstep c, c < ncols (<= 16), of W to L and to the FP32
+  // copy: the diagonal tile's rows of L11 and the solved tile rows.  All LDS
+  // reads are issued before the first store.
+  auto write_back = [&](int c0, int cstep, int ncols) {
+    double vd[16], vt[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      if (c < ncols) {
+        const int col = c0 + cstep * c;  // < 64
+        vd[c] = W[col * kWld + lane];
+        vt[c] = W[col * kWld + 64 + lane];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int col = c0 + cstep * c;
+      if (c < ncols && col < nc) {
+        if (tl.kind == 0 && lane < nc && col <= lane) {
+          gtile[(int64_t)col * tl.ld + lane] = vd[c];
+          if (ftile) ftile[(int64_t)col * tl.ld + lane] = (float)vd[c];
+        }
+        if (lane < nx) {
+          gtile[(int64_t)col * tl.ld + first + lane] = vt[c];
+          if (ftile) ftile[(int64_t)col * tl.ld + first + lane] = (float)vt[c];
+        }
+      }
+    }
+  };
   lds_barrier();
   int badcol = 0;
-  for (int J = 0; J < nb; ++J) {
-    const int jb = 16 * J;
-    // (1) this lane's W row (wave 3 only reads one); the lane holding pivot row
-    // jb + j is pbase + j.  Every wave has read its rows before any is written
-    // back (waves 1 and 2 read the pivot rows that wave 0 rewrites).
-    const int row = wave == 0 ? lane
-                              : (wave == 3 || lane < 16 || (wave == 2 && lane >= 32)
-                                     ? jb + (lane & 15)
-                                     : 64 + 48 * (wave - 1) + (lane - 16));
-    double x[16];
+  // (1) this lane's W row in block column jb/16 (wave 3 only reads one); the
+  // lane holding pivot row jb + j is pbase + j.  Every wave has read its rows
+  // before any is written back (waves 1 and 2 read the pivot rows that wave 0
+  // rewrites): a barrier follows every read.
+  auto sweep_row = [&](int jb) {
+    return wave == 0 ? lane
+                     : (wave == 3 || lane < 16 || (wave == 2 && lane >= 32)
+                            ? jb + (lane & 15)
+                            : 64 + 48 * (wave - 1) + (lane - 16));
+  };
+  double x[16];
+  auto read_row = [&](int jb) {
+    const int row = sweep_row(jb);
 #pragma unroll
     for (int c = 0; c < 16; ++c) x[c] = W[(jb + c) * kWld + row];
-    lds_barrier();
+  };
+  read_row(0);
+  lds_barrier();
+  for (int J = 0; J < nb; ++J) {
+    const int jb = 16 * J;
+    const int row = sweep_row(jb);
     if (wave < 3) {
       const int pbase = wave == 0 ? jb : 0;
       const bool store = wave == 0 ? (lane >= jb && lane < 16 * nb)
@@ -693,15 +737,15 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         // x[j] has every update from the columns before j
-        if (d > 0.0) {
-          // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
-          // by 1/ajj)
-          x[j] *= rsqrt(d);
-        } else {
-          // a non-positive pivot is reported and continues as 1, so no NaNs spread
-          badcol = badcol == 0 ? jb + j + 1 : badcol;
-          x[j] = lane == pbase + j ? 1.0 : x[j];
-        }
+        // 1/sqrt(d) once, then a multiply per entry (dpotf2 scales the column
+        // by 1/ajj).  A non-positive pivot is reported and continues as 1, so
+        // no NaNs spread.  Selects rather than a branch: the test on d runs
+        // beside the reciprocal square root instead of before it.
+        double r = rsqrt(d);
+        asm volatile("" : "+v"(r));  // computed whatever d is, not under a branch
+        const bool pos = d > 0.0;
+        badcol = pos || badcol != 0 ? badcol : jb + j + 1;
+        x[j] = pos ? x[j] * r : (lane == pbase + j ? 1.0 : x[j]);
         if (j < 15) {
           const double m1 = bcast(x[j], pbase + j + 1);  // entry (j+1, j)
           if (j >= 1) x[j + 1] -= x[j - 1] * pm[j + 1];
@@ -721,6 +765,10 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
 #pragma unroll
         for (int c = 0; c < 16; ++c) W[(jb + c) * kWld + row] = x[c];
       }
+    } else if (J > 0) {
+      // block column J - 1 is final (the updates after its sweep touch later
+      // columns only): wave 3 writes it back while the others sweep
+      write_back(jb - 16, 1, 16);
     }
     lds_barrier();
     if (J + 1 < nb) {
@@ -742,27 +790,13 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
         }
       }
       lds_barrier();
+      read_row(jb + 16);
+      lds_barrier();
     }
   }
   if (tl.kind == 0 && badcol && badcol <= nc && tid == 0) atomicMin(info_out, tl.c0 + badcol);
-  float* ftile = Lf ? Lf + tl.vbase + tl.r0 : nullptr;
-  const int c0 = 16 * wave;
-#pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    const int col = c0 + c;
-    if (col < nc) {
-      if (tl.kind == 0 && lane < nc && col <= lane) {
-        const double v = W[col * kWld + lane];
-        gtile[(int64_t)col * tl.ld + lane] = v;
-        if (ftile) ftile[(int64_t)col * tl.ld + lane] = (float)v;
-      }
-      if (lane < nx) {
-        const double v = W[col * kWld + 64 + lane];
-        gtile[(int64_t)col * tl.ld + first + lane] = v;
-        if (ftile) ftile[(int64_t)col * tl.ld + first + lane] = (float)v;
-      }
-    }
-  }
+  // the last block column, four columns per wave
+  write_back(16 * (nb - 1) + wave, 4, 4);
 }
 
 __global__ void __launch_bounds__(256)
