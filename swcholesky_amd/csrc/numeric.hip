@@ -799,12 +799,34 @@ __device__ inline void panel_factor_blk_body(double* __restrict__ L, const Tile&
   write_back(16 * (nb - 1) + wave, 4, 4);
 }
 
+// The tiles of a level's panels of at most 16 columns, four to a workgroup (one
+// wave each, as k_panel_factor_regs<16>), in a launch of 256-thread workgroups
+// whose first ceil(nregs / 4) workgroups take them: the level's panels of both
+// width classes are then factored in one launch.  Returns false for the
+// workgroups after those.
+__device__ inline bool regs_tiles_in_block(double* __restrict__ L,
+                                           const Tile* __restrict__ ftiles, int rlist0,
+                                           int nregs, const double* __restrict__ dscratch,
+                                           int* __restrict__ info_out, float* __restrict__ Lf) {
+  if ((int)blockIdx.x >= (nregs + 3) >> 2) return false;
+  const int t = 4 * (int)blockIdx.x + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  if (t < nregs) {
+    const Tile tl = ftiles[rlist0 + t];
+    panel_factor_regs_body<16>(L, tl, dscratch, info_out, Lf, threadIdx.x & 63);
+  }
+  return true;
+}
+
+// Workgroups [0, ceil(nregs / 4)): the level's panels of at most 16 columns
+// (ftiles[rlist0 ..]); the others: one tile of a 17..64-column panel each
+// (ftiles[list0 ..]).
 __global__ void __launch_bounds__(256)
 k_panel_factor_blk(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0,
                    const double* __restrict__ dscratch, int* __restrict__ info_out,
-                   float* __restrict__ Lf) {
+                   float* __restrict__ Lf, int rlist0, int nregs) {
   __shared__ double lds[kBlkLds];
-  const Tile tl = ftiles[list0 + blockIdx.x];
+  if (regs_tiles_in_block(L, ftiles, rlist0, nregs, dscratch, info_out, Lf)) return;
+  const Tile tl = ftiles[list0 + blockIdx.x - ((nregs + 3) >> 2)];
   panel_factor_blk_body(L, tl, dscratch, info_out, Lf, lds, threadIdx.x);
 }
 
@@ -937,9 +959,10 @@ k_dag_factor(double* __restrict__ L, const Tile* __restrict__ tiles,
   }
 }
 
-// A level's panel factorization (panels of 17..64 columns, workgroups
-// [0, nfac): k_panel_factor_blk) in the same launch as the level's updates of
-// tiles whose panels it does not factor (the remaining workgroups:
+// A level's panel factorization (panels of at most 16 columns in the first
+// ceil(nregs / 4) workgroups, four tiles each; then panels of 17..64 columns,
+// nfac workgroups: k_panel_factor_blk) in the same launch as the level's updates
+// of tiles whose panels it does not factor (the remaining workgroups:
 // k_tile_update), so that those updates, which nothing of this level waits for,
 // overlap the factorization instead of delaying it.  Same bodies, same results.
 template <int MODE>
@@ -947,13 +970,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
 k_factor_and_update(double* __restrict__ L, const Tile* __restrict__ ftiles, int list0, int nfac,
                     const Tile* __restrict__ tiles, const Task* __restrict__ tasks, int tile0,
                     double* __restrict__ dscratch, int* __restrict__ info_out, float* Lf_factor,
-                    const float* __restrict__ Lf_update) {
+                    const float* __restrict__ Lf_update, int rlist0, int nregs) {
   __shared__ double lds[kDagLds];
-  if ((int)blockIdx.x < nfac) {
-    const Tile tl = ftiles[list0 + blockIdx.x];
+  if (regs_tiles_in_block(L, ftiles, rlist0, nregs, dscratch, info_out, Lf_factor)) return;
+  const int b = (int)blockIdx.x - ((nregs + 3) >> 2);
+  if (b < nfac) {
+    const Tile tl = ftiles[list0 + b];
     panel_factor_blk_body(L, tl, dscratch, info_out, Lf_factor, lds, threadIdx.x);
   } else {
-    const Tile tl = tiles[tile0 + (blockIdx.x - nfac)];
+    const Tile tl = tiles[tile0 + (b - nfac)];
     tile_update_body<MODE>(L, tl, tasks, dscratch, Lf_update, lds, threadIdx.x);
   }
 }
@@ -2267,21 +2292,24 @@ void enqueue_factor(swc_handle* h, bool profile) {
       const int* cls = &P.lvl_trsm_cls[(size_t)4 * l];
       float* lf = h->fp32_mask ? h->dLf : nullptr;
       const Tile* ft = (const Tile*)h->dFtiles;
-      const int nblk = ns > 0 ? cls[3] - cls[1] : 0;
-      if (ns > 0 && cls[1] > cls[0])
-        hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(cls[1] - cls[0]), dim3(64), 0, st, h->dL, ft,
-                           cls[0], (const double*)h->dDiag, h->dInfo, lf);
+      const int nblk = ns > 0 ? cls[3] - cls[1] : 0, nreg = ns > 0 ? cls[1] - cls[0] : 0;
+      // the narrow panels go into the launch of the wider ones when there is one
+      // (four tiles to a workgroup), else into their own launch
+      const int nrw = (nreg + 3) / 4;
       if (ntd > 0) {
         auto kern = h->precision == 0
                         ? k_factor_and_update<0>
                         : (h->precision == 1 ? k_factor_and_update<1> : k_factor_and_update<2>);
-        hipLaunchKernelGGL(kern, dim3(nblk + ntd), dim3(kThreads), 0, st, h->dL, ft,
+        hipLaunchKernelGGL(kern, dim3(nrw + nblk + ntd), dim3(kThreads), 0, st, h->dL, ft,
                            ns > 0 ? cls[1] : 0, nblk, (const Tile*)h->dTiles,
                            (const Task*)h->dTasks, tc0 + ntc, h->dDiag, h->dInfo, lf,
-                           (const float*)h->dLf);
+                           (const float*)h->dLf, ns > 0 ? cls[0] : 0, nreg);
       } else if (nblk > 0) {
-        hipLaunchKernelGGL(k_panel_factor_blk, dim3(nblk), dim3(256), 0, st, h->dL, ft,
-                           cls[1], (const double*)h->dDiag, h->dInfo, lf);
+        hipLaunchKernelGGL(k_panel_factor_blk, dim3(nrw + nblk), dim3(256), 0, st, h->dL, ft,
+                           cls[1], (const double*)h->dDiag, h->dInfo, lf, cls[0], nreg);
+      } else if (nreg > 0) {
+        hipLaunchKernelGGL(k_panel_factor_regs<16>, dim3(nreg), dim3(64), 0, st, h->dL, ft,
+                           cls[0], (const double*)h->dDiag, h->dInfo, lf);
       }
     }
     if (pe) (void)hipEventRecord(pe[2], st);
