@@ -117,8 +117,8 @@ def main():
                          "copy kernel of known size, to scale HBM counters)")
     ap.add_argument("--params", default=None,
                     help="plan shape parameters as comma-separated integers {rlb, spw, "
-                         "narrow_min_tiles, narrow_max_k, fuse_narrow, dag} (default: the "
-                         "tuning table's entry for the structure)")
+                         "narrow_min_tiles, narrow_max_k, fuse_narrow, dag}, dag = 0 (default: "
+                         "the tuning table's entry for the structure)")
     args = ap.parse_args()
     args.plan = tuple(int(v) for v in args.params.split(",")) if args.params else "auto"
     if args.gpus != 1:

@@ -143,10 +143,10 @@ int swc_set_perm(swc_handle* h, const int* perm);
  * or 128 columns per solve panel, the one-wave tile kernel for levels with at least
  * narrow_min_tiles tiles of panels <= 16 columns whose inner dimensions sum to
  * <= narrow_max_k, with fuse_narrow = 1 such panels (all tiles narrow) updated
- * and factored by one wave each in one persistent launch per level, and with
- * dag = 1 (list-schedule order) or 2 (level order) the levels after those as one
- * dependency-driven persistent launch (bitwise the same factor; a bounded wait
- * that runs out makes swc_factor fail with SWC_EDEVICE).  No reference counterpart (the reference's Sunway BLAS fixes
+ * and factored by one wave each in one persistent launch per level.  dag must be
+ * 0: the dependency-driven launch of the later levels (dag = 1, 2) is disabled and
+ * rejected with SWC_EINVAL, after one run of it gave a wrong factor whose cause
+ * was not found.  No reference counterpart (the reference's Sunway BLAS fixes
  * its blocking, BLAS-sw/dpotrf_cluster_master.c:11-63); the values come from the
  * autotuner's table (swcholesky_amd/tuning.json). */
 int swc_set_plan_params(swc_handle* h, const int* params, int count);
@@ -273,8 +273,10 @@ typedef struct swc_plan swc_plan;
 int swc_plan_build(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                    const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                    swc_plan** out);
-/* The same with plan shape parameters (as swc_set_plan_params) and, with
- * nLevels > 0, the H-level schedule (as swc_set_schedule). */
+/* The same with plan shape parameters (as swc_set_plan_params, but dag = 1 or 2 is
+ * accepted here: the plan's dependency-driven work items can be inspected, although
+ * the launch that would run them is disabled) and, with nLevels > 0, the H-level
+ * schedule (as swc_set_schedule). */
 int swc_plan_build_params(int n, const int* blockSet, int supNo, const size_t* lC, const int* Ls,
                           const size_t* Li_ptr, const int* prunePtr, const int* pruneSet,
                           const int* params, int nparams, int nLevels, const int* levelPtr,

@@ -3282,6 +3282,12 @@ int swc_set_plan_params(swc_handle* h, const int* params, int count) {
   swc::PlanParams p;
   if (params && !swc::plan_params_from(params, count, p))
     return swc_set_error(SWC_EINVAL, "swc_set_plan_params: parameter out of range");
+  // The dependency-driven launch is switched off: one run of it gave a wrong factor
+  // (a non-positive pivot where the per-level launches give none) without a cause
+  // found in its waits and fences.
+  if (p.dag != 0)
+    return swc_set_error(SWC_EINVAL,
+                         "swc_set_plan_params: the dependency-driven launch (dag) is disabled");
   h->params = p;  // NULL: the defaults
   return SWC_OK;
 }

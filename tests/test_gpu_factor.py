@@ -345,7 +345,14 @@ def test_fused_narrow_panels_match_two_kernel_path(name, make, mode):
 
 DAG_CASES = [SMALL[2], SMALL[3], SMALL[4], SMALL[6], ("lap3d_var_32", lambda: P.laplacian3d_var(32))]
 
+# The dependency-driven launch is switched off (swc_set_plan_params rejects dag != 0):
+# one run of the test below gave a non-positive pivot (info 32549 on lap3d_var_32,
+# mode 1, order 1) where the per-level launches give none, and no cause was found in
+# the launch's waits and fences.
+DAG_OFF = "dependency-driven launch disabled: an intermittent wrong factor, cause not found"
 
+
+@pytest.mark.skip(reason=DAG_OFF)
 @pytest.mark.parametrize("name,make", DAG_CASES, ids=[c[0] for c in DAG_CASES])
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_dependency_driven_launch_matches_level_launches(name, make, mode):
@@ -371,6 +378,7 @@ def test_dependency_driven_launch_matches_level_launches(name, make, mode):
             assert np.array_equal(f.L_values(), Lref), (params, rep)
 
 
+@pytest.mark.skip(reason=DAG_OFF)
 def test_dependency_driven_launch_reports_non_positive_pivot():
     prob = P.laplacian2d_var(20, 5, shift=-2.0)  # strongly indefinite
     b, a2x, Lo, info_o = setup(prob)
@@ -378,6 +386,15 @@ def test_dependency_driven_launch_reports_non_positive_pivot():
     for params in ((8, 128, 1 << 30, 256, 0, 1), (8, 128, 1 << 30, 256, 0, 0)):
         f = S.Factor(b, params=params)
         assert f.factor(prob.values) == info_o
+
+
+def test_dependency_driven_launch_is_rejected():
+    """swc_set_plan_params refuses dag = 1 or 2 with an error naming the launch."""
+    prob = P.laplacian3d_var(8)
+    b = S.analyze_p2(prob.n, prob.colptr, prob.rowidx, prob.perm)
+    for dag in (1, 2):
+        with pytest.raises(Exception, match="dependency-driven"):
+            S.Factor(b, params=(8, 128, 1024, 256, 1, dag))
 
 
 @pytest.mark.parametrize("name,make", [("dense_2000", lambda: P.dense_spd(2000))],
