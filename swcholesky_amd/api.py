@@ -171,7 +171,8 @@ class Factor:
     def __init__(self, bcsc: BCSC, params="auto", schedule=None):
         """params: "auto" (the autotuner's table entry for this structure's class,
         swcholesky_amd/tuning.py), None (the library defaults) or a sequence
-        {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag} (swc_set_plan_params).
+        {rlb, spw, narrow_min_tiles, narrow_max_k, fuse_narrow, dag} (swc_set_plan_params;
+        dag must be 0: the dependency-driven launch is disabled).
         schedule: None (wavefront levels) or the reference's H-level arrays
         (levelPtr, parPtr, partition) (swc_set_schedule)."""
         self.bcsc = bcsc
