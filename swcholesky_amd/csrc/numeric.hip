@@ -1453,8 +1453,9 @@ k_bwd_partial(const double* __restrict__ L, const int* __restrict__ Ls,
 
 // t = y - (sum of the panel's row-block partials) for RB right-hand sides, into
 // ts: thread t sums half t / SPW of the blocks for column t mod SPW, in block
-// order, 16 loads in flight per step; the halves are added first half first.
+// order, kGather loads in flight per step; the halves are added first half first.
 // Every column of ts is written (zero past nc and for padded right-hand sides).
+constexpr int kGather = 32;
 template <int RB>
 __device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& sp, int f0,
                                        int f1, int b0, const double* __restrict__ Pb, int64_t pld,
@@ -1464,22 +1465,22 @@ __device__ inline void gather_partials(double (&ts)[RB][swc::SPW], const Panel& 
   const int fm = f0 + (f1 - f0 + 1) / 2, fa = h == 0 ? f0 : fm, fb = h == 0 ? fm : f1;
   double sum[RB] = {};
   if (c < nc) {
-    for (int b = fa; b < fb; b += 16) {
+    for (int b = fa; b < fb; b += kGather) {
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
         if (q < nq) {  // uniform
           const double* pb = Pb + (int64_t)(rb0 + q) * pld + c;
-          double pv[16];
+          double pv[kGather];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) {
+          for (int u = 0; u < kGather; ++u) {
             // clamped to the last block, so every load is inside the partials
             const double val = pb[(int64_t)(min(b + u, fb - 1) - b0) * swc::SPW];
             pv[u] = b + u < fb ? val : 0.0;
           }
 #pragma unroll
-          for (int u = 0; u < 16; ++u) sum[q] += pv[u];
+          for (int u = 0; u < kGather; ++u) sum[q] += pv[u];
         }
-        __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight, not 16 RB
+        __builtin_amdgcn_sched_barrier(0);  // kGather loads in flight, not kGather RB
       }
     }
   }
